@@ -1,0 +1,135 @@
+"""Benchmark: histories/s of the k-eigenvalue solve on one MI355X.
+
+Workload (the configuration the headline metric is quoted on): the 9-region
+4-group slab (5 fuel regions of 1.0 cm, 4 water regions of 0.5 cm, 20 cells
+per region, reflective left / vacuum right), 10^7 particles per generation,
+20 inactive + 100 active generations, seeded synthetic cross sections.
+
+A step is one full k-eigenvalue solve of that workload (120 generations).
+`value` is the FP32 event-based (Woodcock) histories/s over the K timed
+steps; the plain FP64 port (one thread per history, surface tracking) runs
+the same workload once beside it, and the FP64 CPU oracle is timed on a
+bounded sample on the host cores.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+B_STATE_FP32, B_STATE_FP64 = 24, 32
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 port leg")
+    ap.add_argument("--cpu-histories", type=int, default=1_000_000)
+    ap.add_argument("--cpu-generations", type=int, default=12)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+
+    import nraps_amd as nr
+    from nraps_amd import _abi
+
+    st, info = nr.device_info(0)
+    if st != 0:
+        raise SystemExit("no HIP device: " + info)
+    with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
+        fbar = json.load(f)["workloads"]["c1"]["mean_flights"]
+
+    prob, o = nr.generate(1)           # 10^7 x (20 + 100), FP32 event Woodcock
+    o.use_graph = 0                    # per-generation HIP events around the transport kernel
+    N, gens = int(o.histories), int(o.generations)
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    for i in range(args.warmup):
+        nr.solve(prob, o)
+        log(f"warmup {i + 1}/{args.warmup} done")
+    tr_ms, tr_launch, ks = 0.0, 0, []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        r = nr.solve(prob, o)
+        tr_ms += r.ms_transport
+        tr_launch += r.transport_launches
+        ks.append((r.k_mean, r.k_sigma))
+        log(f"step {i + 1}/{args.steps}: k = {r.k_mean:.6f} +- {r.k_sigma:.6f}")
+    t1 = time.perf_counter()
+    wall = t1 - t0
+    value = args.steps * N * gens / wall
+    k32, s32 = ks[-1]
+
+    out = {
+        "metric": "histories/s on 1 GPU (FP32 vs plain FP64) at k within 3σ; p50/p99 latency",
+        "value": value, "unit": "histories/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "9-region 4-group slab (c1): 1e7 particles/generation, "
+                               "20 inactive + 100 active generations, FP32 event-based "
+                               "Woodcock tracking, FP64 k", "histories_per_generation": N,
+                   "generations": gens, "cells": prob.n_cells, "groups": prob.n_groups},
+        "k_fp32": k32, "k_fp32_sigma": s32,
+    }
+    # roofline of the dominant kernel (the transport launch)
+    b_hist = B_STATE_FP32 * (1 + 2 * fbar)
+    avg_ms = tr_ms / max(tr_launch, 1)
+    achieved = b_hist * N / (avg_ms * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "kernel": "k_woodcock<float,lds>", "avg_launch_ms": avg_ms,
+                       "bytes_per_history": b_hist, "mean_flights": fbar}
+
+    if not args.no_fp64:
+        o64 = nr.generate(1)[1]
+        o64.method, o64.precision, o64.use_graph = _abi.HISTORY_SURFACE, _abi.FP64, 0
+        t = time.perf_counter()
+        r64 = nr.solve(prob, o64)
+        w64 = time.perf_counter() - t
+        v64 = N * gens / w64
+        comb = math.hypot(s32, r64.k_sigma)
+        out["fp64_plain_histories_per_s"] = v64
+        out["speedup_vs_fp64_plain"] = value / v64
+        out["k_fp64_plain"] = r64.k_mean
+        out["k_fp64_plain_sigma"] = r64.k_sigma
+        out["k_delta_over_sigma"] = abs(k32 - r64.k_mean) / comb if comb > 0 else None
+        log(f"plain FP64 port: {v64:.4g} histories/s, k = {r64.k_mean:.6f} +- {r64.k_sigma:.6f}")
+
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        threads = max((os.cpu_count() or 2) - 1, 1)
+        threads = min(threads, 15)
+        oc = nr.run_opts(histories=args.cpu_histories, generations=args.cpu_generations, skip=2)
+        t = time.perf_counter()
+        rc = oracle_lib.solve(prob, oc, threads=threads)
+        wc = time.perf_counter() - t
+        cpu_v = args.cpu_histories * args.cpu_generations / wc
+        out["cpu_baseline"] = {"value": cpu_v, "unit": "histories/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"FP64 CPU oracle (reference semantics, surface tracking) "
+                                         f"on the c1 problem: {args.cpu_histories} histories x "
+                                         f"{args.cpu_generations} generations, {wc:.1f} s"}
+        out["speedup_vs_cpu"] = value / cpu_v
+        log(f"CPU oracle: {cpu_v:.4g} histories/s on {threads} threads")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,113 @@
+// runtime.h — host runtime objects shared by the solver and the server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <vector>
+
+#include "device.h"
+#include "host.h"
+
+namespace nraps {
+
+struct DeviceError : std::runtime_error {
+  hipError_t err;
+  DeviceError(const char *what, hipError_t e);
+};
+
+#define NRAPS_HIP_TRY(x)                                     \
+  do {                                                       \
+    hipError_t e_ = (x);                                     \
+    if (e_ != hipSuccess) throw ::nraps::DeviceError(#x, e_); \
+  } while (0)
+
+// Owns device allocations; frees them on destruction.
+class DevArena {
+ public:
+  DevArena() = default;
+  DevArena(const DevArena &) = delete;
+  DevArena &operator=(const DevArena &) = delete;
+  ~DevArena();
+  void *raw(size_t bytes);
+  template <class T>
+  T *alloc(size_t n) {
+    return static_cast<T *>(raw(n * sizeof(T)));
+  }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  std::vector<void *> ptrs_;
+  size_t bytes_ = 0;
+};
+
+int ktree_levels(int64_t n);
+int64_t site_capacity(int64_t N);
+// Device bytes held for one problem (banks, tallies, accumulators).
+int64_t problem_footprint(const nraps_problem *pb, const nraps_run_opts *o);
+
+// One problem's device state: tables, banks, tallies, accumulators.
+class ProblemDev {
+ public:
+  // cs_slot: externally owned CycleState (server slot array) or nullptr.
+  void setup(const nraps_problem *pb, const nraps_run_opts &o, bool fp32, bool woodcock,
+             CycleState *cs_slot, bool want_trace, nraps_result *trace_res);
+  // Copy results into r; returns the status of the run.
+  int download(nraps_result *r, double ms_total, double ms_device, double ms_transport,
+               int64_t launches);
+  // Stop the problem from taking part in further launches.
+  void deactivate(hipStream_t s);
+
+  const nraps_problem *pb = nullptr;
+  nraps_run_opts o{};
+  Prepared prep;
+  std::vector<uint8_t> blob;
+  TabLayout lay{};
+  bool fission = false;
+  int64_t trace_n = 0;
+  DevProblem dev{};
+  DevFinal fin{};
+  CycleState *d_cs = nullptr;
+  double *d_kref = nullptr, *d_flux_acc = nullptr, *d_fs_acc = nullptr, *d_fsum = nullptr,
+         *d_fsq = nullptr, *d_tally_last = nullptr;
+  int64_t *d_sites = nullptr;
+  unsigned long long *d_stats = nullptr;
+  DevArena arena;
+};
+
+// Launch tables and parameters for a set of problems run together.
+class Launcher {
+ public:
+  explicit Launcher(int device);
+  ~Launcher();
+  void build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodcock);
+  // Launches of one generation for every problem of the set.
+  void enqueue_cycle(hipStream_t s, hipEvent_t tr0, hipEvent_t tr1);
+  // Capture one generation into a graph (replaces any previous capture).
+  void capture(hipStream_t s);
+  void launch_graph(hipStream_t s);
+  bool has_graph() const { return graph_exec_ != nullptr; }
+  int grid() const { return grid_; }
+  bool lds_tally() const { return lds_tally_; }
+
+ private:
+  void drop_graph();
+  int device_;
+  int n_ = 0, n_cu_ = 0, lds_limit_ = 0, blocks_per_cu_ = 1, grid_ = 1, claim_ = 256;
+  int tab_lds_ = 0, max_bins_ = 0, levels_ = 1;
+  int64_t max_n_ = 0, max_tiles_ = 0;
+  size_t lds_bytes_ = 0;
+  bool fp32_ = false, woodcock_ = false, fission_any_ = false, lds_tally_ = false;
+  DevProblem *d_probs_ = nullptr;
+  DevFinal *d_fins_ = nullptr;
+  int64_t *d_tile_sums_ = nullptr, *d_tile_base_ = nullptr;
+  int64_t cap_probs_ = 0, cap_tiles_ = 0;
+  std::vector<void *> owned_;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+};
+
+int solve_batch(int n, const nraps_problem *const *pbs, const nraps_run_opts *opts,
+                nraps_result *const *results);
+
+}  // namespace nraps

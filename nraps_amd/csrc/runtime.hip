@@ -1,0 +1,603 @@
+// runtime.hip — host runtime of the MI355X solver: per-problem device
+// state, the per-generation launch sequence (optionally captured once in a
+// hipGraph and replayed), and the download of results into the caller's
+// arrays.
+//
+// nraps_mc_solve() replaces the body of monte_carlo (src/mc_code.rs:276-380):
+// same inputs, same SolutionResults arrays, same normalisation.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "launch.h"
+#include "runtime.h"
+
+namespace nraps {
+
+DeviceError::DeviceError(const char *what, hipError_t e)
+    : std::runtime_error(std::string(what) + ": " + hipGetErrorString(e)), err(e) {}
+
+#define HIP_TRY NRAPS_HIP_TRY
+
+DevArena::~DevArena() {
+  for (void *p : ptrs_) (void)hipFree(p);
+}
+
+void *DevArena::raw(size_t bytes) {
+  if (bytes == 0) bytes = 16;
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) throw DeviceError("hipMalloc", e);
+  ptrs_.push_back(p);
+  bytes_ += bytes;
+  return p;
+}
+
+static int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+int ktree_levels(int64_t n) {
+  int L = 0;
+  do {
+    n = ceil_div(n, kTreeWidth);
+    L++;
+  } while (n > 1);
+  return L;
+}
+
+int64_t site_capacity(int64_t N) { return 4 * N + 65536; }
+
+int64_t problem_footprint(const nraps_problem *pb, const nraps_run_opts *o) {
+  const int64_t N = o->histories, nc = pb->n_cells, G = pb->n_groups;
+  const int rs = o->precision == NRAPS_FP32 ? 4 : 8;
+  int64_t b = 0;
+  b += N * 8;                                  // kacc
+  b += 2 * G * nc * 8;                         // generation tally
+  b += 4 * G * nc * 8 + 4 * nc * 8;            // accumulators, finalize tables
+  b += (ceil_div(N, kTreeWidth) + ceil_div(N, 65536) + 2) * 8;
+  b += (int64_t)o->generations * 24;
+  if (o->source == NRAPS_SOURCE_FISSION_BANK) {
+    int64_t cap = site_capacity(N);
+    b += N * (4 + 8 + rs + 4);                 // counts, offsets, source bank
+    b += cap * (rs + 4 + 8) + cap * (rs + 4);  // unordered + ordered sites
+    b += ceil_div(N, kScanTile) * 8;
+  }
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// ProblemDev
+// ---------------------------------------------------------------------------
+void ProblemDev::setup(const nraps_problem *pb_, const nraps_run_opts &o_, bool fp32, bool woodcock,
+                       CycleState *cs_slot, bool want_trace, nraps_result *r) {
+  pb = pb_;
+  o = o_;
+  prep.build(pb);
+  blob = prep.pack(fp32, &lay);
+  fission = o.source == NRAPS_SOURCE_FISSION_BANK;
+  const Prepared &P = prep;
+  const int64_t N = o.histories;
+  const int rs = fp32 ? 4 : 8;
+  DevProblem &d = dev;
+  std::memset(&d, 0, sizeof d);
+  uint8_t *tab = arena.alloc<uint8_t>(blob.size());
+  HIP_TRY(hipMemcpy(tab, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  d.tab = tab;
+  d.lay = lay;
+  d.nc = P.nc;
+  d.G = P.G;
+  d.M = P.M;
+  d.R = P.R;
+  d.nf = P.nf;
+  d.xs_every = o.xs_refresh == NRAPS_XS_EVERY_COLLISION;
+  d.fission = fission;
+  d.tally_bins = 2 * P.G * P.nc;
+  d.N = N;
+  d.seed = o.seed;
+  d.problem_id = o.problem_id;
+  d.max_events = o.max_events > 0 ? o.max_events : 10000000;
+  d.boundl = P.boundl;
+  d.boundr = P.boundr;
+  d.XL = P.XL;
+  d.XR = P.XR;
+  d.maj = P.maj;
+  d.inv_maj = 1.0 / P.maj;
+  CycleState cs;
+  std::memset(&cs, 0, sizeof cs);
+  cs.k_prev = o.k_initial;
+  cs.active = 1;
+  d_cs = cs_slot ? cs_slot : arena.alloc<CycleState>(1);
+  HIP_TRY(hipMemcpy(d_cs, &cs, sizeof cs, hipMemcpyHostToDevice));
+  d.cs = d_cs;
+  d.k = arena.alloc<double>(o.generations);
+  d.kacc = arena.alloc<double>(N);
+  d.tally_glob = arena.alloc<double>(d.tally_bins);
+  if (fission) {
+    d.counts = arena.alloc<int32_t>(N);
+    d.offsets = arena.alloc<int64_t>(N);
+    d.site_cap = site_capacity(N);
+    d.site_x = arena.raw((size_t)d.site_cap * rs);
+    d.site_cell = arena.alloc<int32_t>(d.site_cap);
+    d.site_key = arena.alloc<uint64_t>(d.site_cap);
+    d.ord_x = arena.raw((size_t)d.site_cap * rs);
+    d.ord_cell = arena.alloc<int32_t>(d.site_cap);
+    d.src_x = arena.raw((size_t)N * rs);
+    d.src_cell = arena.alloc<int32_t>(N);
+  }
+  d.tree_a = arena.alloc<double>(ceil_div(N, kTreeWidth) + 1);
+  d.tree_b = arena.alloc<double>(ceil_div(N, 65536) + 1);
+  trace_n = 0;
+  if (want_trace && o.trace_histories > 0 && r && r->trace && r->trace_len && r->draws && r->kacc) {
+    trace_n = o.trace_histories;
+    d.trace = arena.alloc<nraps_trace_event>(trace_n * o.trace_events);
+    d.trace_len = arena.alloc<int32_t>(trace_n);
+    d.draws = arena.alloc<int64_t>(trace_n);
+    d.trace_kacc = arena.alloc<double>(trace_n);
+    HIP_TRY(hipMemset(d.trace_len, 0, sizeof(int32_t) * trace_n));
+    d.trace_n = trace_n;
+    d.trace_ev = o.trace_events;
+    d.trace_cycle = o.trace_cycle;
+  }
+  d_stats = arena.alloc<unsigned long long>(kNumStats);
+  HIP_TRY(hipMemset(d_stats, 0, sizeof(unsigned long long) * kNumStats));
+  d.stats = d_stats;
+
+  DevFinal &f = fin;
+  std::memset(&f, 0, sizeof f);
+  const int nc = P.nc, GM = P.G * P.M, Gnc = P.G * P.nc;
+  std::vector<double> width(nc);
+  for (int c = 0; c < nc; c++) width[c] = pb->mesh_right[c] - pb->mesh_left[c];
+  auto up_d = [&](const double *src, size_t cnt) {
+    double *p = arena.alloc<double>(cnt);
+    HIP_TRY(hipMemcpy(p, src, cnt * 8, hipMemcpyHostToDevice));
+    return p;
+  };
+  f.dx = up_d(pb->delta_x, nc);
+  f.width = up_d(width.data(), nc);
+  f.nut = up_d(pb->nut, GM);
+  f.sigf = up_d(pb->sigf, GM);
+  int32_t *mat = arena.alloc<int32_t>(nc);
+  HIP_TRY(hipMemcpy(mat, pb->matid, nc * 4, hipMemcpyHostToDevice));
+  f.mat = mat;
+  f.nut_conv = P.nut_conv;
+  f.length = P.XR;
+  f.generations = o.generations;
+  f.skip = o.skip;
+  f.tally_cycle = d.tally_glob;
+  f.tally_last = d_tally_last = arena.alloc<double>(Gnc);
+  f.flux_acc = d_flux_acc = arena.alloc<double>(Gnc);
+  f.fs_acc = d_fs_acc = arena.alloc<double>(nc);
+  f.fsum = d_fsum = arena.alloc<double>(Gnc);
+  f.fsq = d_fsq = arena.alloc<double>(Gnc);
+  f.k_ref = d_kref = arena.alloc<double>(o.generations);
+  f.sites_hist = d_sites = arena.alloc<int64_t>(o.generations);
+  HIP_TRY(hipMemset(d_flux_acc, 0, Gnc * 8));
+  HIP_TRY(hipMemset(d_fs_acc, 0, nc * 8));
+  HIP_TRY(hipMemset(d_fsum, 0, Gnc * 8));
+  HIP_TRY(hipMemset(d_fsq, 0, Gnc * 8));
+  HIP_TRY(hipMemset(d_tally_last, 0, Gnc * 8));
+  HIP_TRY(hipMemset(d.k, 0, o.generations * 8));
+  HIP_TRY(hipMemset(d_kref, 0, o.generations * 8));
+  HIP_TRY(hipMemset(d_sites, 0, o.generations * 8));
+  f.has_diff = woodcock ? 1 : 0;
+}
+
+void ProblemDev::deactivate(hipStream_t s) {
+  const int32_t zero = 0;
+  HIP_TRY(hipMemcpyAsync(&d_cs->active, &zero, sizeof zero, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+}
+
+int ProblemDev::download(nraps_result *r, double ms_total, double ms_device, double ms_transport,
+                         int64_t launches) {
+  const Prepared &P = prep;
+  const int nc = P.nc, G = P.G, gens = o.generations, Gnc = G * nc;
+  CycleState cs;
+  HIP_TRY(hipMemcpy(&cs, d_cs, sizeof cs, hipMemcpyDeviceToHost));
+  // generations actually run (the server may stop a problem early)
+  const int ran = std::min(gens, (int)cs.cycle);
+  const int skip = o.skip;
+  std::vector<double> k(gens), kref(gens), flux(Gnc), fs(nc), fsum(Gnc), fsq(Gnc), tl(Gnc);
+  std::vector<int64_t> sites(gens);
+  unsigned long long st[kNumStats];
+  HIP_TRY(hipMemcpy(k.data(), dev.k, gens * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(kref.data(), d_kref, gens * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(sites.data(), d_sites, gens * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(flux.data(), d_flux_acc, Gnc * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(fs.data(), d_fs_acc, nc * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(fsum.data(), d_fsum, Gnc * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(fsq.data(), d_fsq, Gnc * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(tl.data(), d_tally_last, Gnc * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(st, d_stats, sizeof st, hipMemcpyDeviceToHost));
+  if (!r) return NRAPS_OK;
+  auto put = [](double *dst, const std::vector<double> &v) {
+    if (dst) std::memcpy(dst, v.data(), v.size() * 8);
+  };
+  put(r->k, k);
+  put(r->k_ref, kref);
+  put(r->flux, flux);
+  put(r->fission_source, fs);
+  put(r->tally_last, tl);
+  if (r->sites) std::memcpy(r->sites, sites.data(), gens * 8);
+  // average_assembly, src/mc_code.rs:259-274
+  if (r->assembly_average) {
+    std::fill(r->assembly_average, r->assembly_average + Gnc, 0.0);
+    const int na = P.numass, per = nc / na;
+    for (int g = 0; g < G && per > 0; g++)
+      for (int a = 0; a < na; a++) {
+        double s = 0.0;
+        for (int c = a * per; c < (a + 1) * per; c++) s += flux[g * nc + c];
+        double avg = s / (double)per;
+        for (int c = a * per; c < (a + 1) * per; c++) r->assembly_average[g * nc + c] = avg;
+      }
+  }
+  // k_fund, src/mc_code.rs:368-376
+  if (r->k_fund) {
+    std::fill(r->k_fund, r->k_fund + gens, 0.0);
+    for (int gg = skip; gg < ran; gg++) {
+      double s = 0.0;
+      for (int x = skip; x <= gg; x++) s += k[x];
+      r->k_fund[gg] = s / (double)(gg - (skip - 1));
+    }
+  }
+  const int nact = std::max(ran - skip, 0);
+  for (int j = 0; j < Gnc; j++) {
+    double mean = nact > 0 ? fsum[j] / nact : 0.0;
+    double var = nact > 1 ? (fsq[j] - fsum[j] * fsum[j] / nact) / (nact - 1) : 0.0;
+    if (var < 0) var = 0;
+    if (r->flux_mean) r->flux_mean[j] = mean;
+    if (r->flux_sigma) r->flux_sigma[j] = nact > 0 ? std::sqrt(var / nact) : 0.0;
+  }
+  double s = 0, s2 = 0;
+  for (int x = skip; x < ran; x++) {
+    s += k[x];
+    s2 += k[x] * k[x];
+  }
+  r->k_mean = nact > 0 ? s / nact : 0.0;
+  double var = nact > 1 ? (s2 - s * s / nact) / (nact - 1) : 0.0;
+  r->k_sigma = var > 0 ? std::sqrt(var / nact) : 0.0;
+  r->histories_run = o.histories * (int64_t)ran;
+  r->collisions = (int64_t)st[kStCollisions];
+  r->virtual_collisions = (int64_t)st[kStVirtual];
+  r->crossings = (int64_t)st[kStCrossings];
+  r->absorptions = (int64_t)st[kStAbsorb];
+  r->leaks = (int64_t)st[kStLeak];
+  r->capped = (int64_t)st[kStCapped];
+  r->mean_flights =
+      r->histories_run > 0 ? (double)(r->collisions + r->leaks) / (double)r->histories_run : 0.0;
+  r->ms_total = ms_total;
+  r->ms_device = ms_device;
+  r->ms_transport = ms_transport;
+  r->transport_launches = launches;
+  if (trace_n > 0) {
+    HIP_TRY(hipMemcpy(r->trace, dev.trace, sizeof(nraps_trace_event) * trace_n * o.trace_events,
+                      hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(r->trace_len, dev.trace_len, 4 * trace_n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(r->draws, dev.draws, 8 * trace_n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(r->kacc, dev.trace_kacc, 8 * trace_n, hipMemcpyDeviceToHost));
+  }
+  int status = NRAPS_OK;
+  if (cs.error & 1)
+    status = NRAPS_E_BANK_OVERFLOW;
+  else if (cs.error & 2)
+    status = NRAPS_E_NO_FISSION;
+  else if (r->capped > 0)
+    status = NRAPS_E_EVENT_CAP;
+  r->status = status;
+  if (status == NRAPS_E_BANK_OVERFLOW) set_error("fission sites exceeded the bank capacity");
+  if (status == NRAPS_E_NO_FISSION) set_error("a generation banked no fission sites");
+  if (status == NRAPS_E_EVENT_CAP) set_error("histories stopped at max_events");
+  return status;
+}
+
+// ---------------------------------------------------------------------------
+// Launcher
+// ---------------------------------------------------------------------------
+Launcher::Launcher(int device) : device_(device) {
+  hipDeviceProp_t prop;
+  HIP_TRY(hipGetDeviceProperties(&prop, device_));
+  n_cu_ = prop.multiProcessorCount;
+  HIP_TRY(hipDeviceGetAttribute(&lds_limit_, hipDeviceAttributeMaxSharedMemoryPerBlock, device_));
+}
+
+Launcher::~Launcher() {
+  drop_graph();
+  for (void *p : owned_) (void)hipFree(p);
+}
+
+void Launcher::drop_graph() {
+  if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+  if (graph_) (void)hipGraphDestroy(graph_);
+  graph_exec_ = nullptr;
+  graph_ = nullptr;
+}
+
+void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodcock) {
+  drop_graph();
+  n_ = (int)ps.size();
+  fp32_ = fp32;
+  woodcock_ = woodcock;
+  fission_any_ = false;
+  max_n_ = 0;
+  max_bins_ = 0;
+  levels_ = 1;
+  max_tiles_ = 0;
+  int tab_max = 0, tally_max = 0;
+  int64_t tiles_total = 0, total = 0;
+  std::vector<int64_t> tile_base(std::max(n_, 1));
+  for (int i = 0; i < n_; i++) {
+    const ProblemDev &p = *ps[i];
+    fission_any_ |= p.fission;
+    max_n_ = std::max<int64_t>(max_n_, p.o.histories);
+    max_bins_ = std::max(max_bins_, p.dev.tally_bins);
+    levels_ = std::max(levels_, ktree_levels(p.o.histories));
+    tile_base[i] = tiles_total;
+    const int64_t t = ceil_div(p.o.histories, kScanTile);
+    tiles_total += t;
+    max_tiles_ = std::max(max_tiles_, t);
+    tab_max = std::max(tab_max, p.lay.bytes);
+    tally_max = std::max(tally_max, p.dev.tally_bins * (fp32 ? 4 : 8));
+    total += p.o.histories;
+  }
+  if (woodcock_ && n_ > 0) {
+    // LDS plan: tables always; the tally too when both leave room for at
+    // least two resident blocks per CU.
+    if (tab_max > lds_limit_)
+      throw std::invalid_argument("problem tables (" + std::to_string(tab_max) +
+                                  " B) exceed the LDS of one block");
+    const int lds_budget = std::min(lds_limit_, 64 * 1024);
+    tab_lds_ = tab_max;
+    lds_tally_ = tab_max + tally_max <= lds_budget;
+    lds_bytes_ = (size_t)tab_max + (lds_tally_ ? (size_t)tally_max : 0);
+    (void)woodcock_set_lds_limit(fp32_, lds_tally_, lds_bytes_);
+    blocks_per_cu_ = woodcock_blocks_per_cu(fp32_, lds_tally_, lds_bytes_);
+    grid_ = n_cu_ * blocks_per_cu_;
+    const int64_t waves = (int64_t)grid_ * (kBlock / 64);
+    int64_t c = total / (waves * 8);
+    c = std::max<int64_t>(64, std::min<int64_t>(1024, c));
+    claim_ = (int)((c + 63) / 64 * 64);
+    if (n_ == 1 && ps[0]->o.chunk > 0) claim_ = (int)std::min<int64_t>(ps[0]->o.chunk, 1 << 20);
+  }
+  if (n_ > cap_probs_) {
+    if (d_probs_) (void)hipFree(d_probs_);
+    if (d_fins_) (void)hipFree(d_fins_);
+    if (d_tile_base_) (void)hipFree(d_tile_base_);
+    owned_.erase(std::remove_if(owned_.begin(), owned_.end(),
+                                [&](void *q) { return q == d_probs_ || q == d_fins_ || q == d_tile_base_; }),
+                 owned_.end());
+    cap_probs_ = std::max<int64_t>(n_, 2 * cap_probs_);
+    HIP_TRY(hipMalloc(&d_probs_, sizeof(DevProblem) * cap_probs_));
+    HIP_TRY(hipMalloc(&d_fins_, sizeof(DevFinal) * cap_probs_));
+    HIP_TRY(hipMalloc(&d_tile_base_, sizeof(int64_t) * cap_probs_));
+    owned_.push_back(d_probs_);
+    owned_.push_back(d_fins_);
+    owned_.push_back(d_tile_base_);
+  }
+  if (tiles_total > cap_tiles_ || !d_tile_sums_) {
+    if (d_tile_sums_) {
+      (void)hipFree(d_tile_sums_);
+      owned_.erase(std::remove(owned_.begin(), owned_.end(), (void *)d_tile_sums_), owned_.end());
+    }
+    cap_tiles_ = std::max<int64_t>(std::max<int64_t>(tiles_total, 1), 2 * cap_tiles_);
+    HIP_TRY(hipMalloc(&d_tile_sums_, sizeof(int64_t) * cap_tiles_));
+    owned_.push_back(d_tile_sums_);
+  }
+  std::vector<DevProblem> dp(n_);
+  std::vector<DevFinal> df(n_);
+  for (int i = 0; i < n_; i++) {
+    dp[i] = ps[i]->dev;
+    df[i] = ps[i]->fin;
+  }
+  if (n_ > 0) {
+    HIP_TRY(hipMemcpy(d_probs_, dp.data(), sizeof(DevProblem) * n_, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_fins_, df.data(), sizeof(DevFinal) * n_, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d_tile_base_, tile_base.data(), sizeof(int64_t) * n_, hipMemcpyHostToDevice));
+  }
+}
+
+void Launcher::enqueue_cycle(hipStream_t s, hipEvent_t tr0, hipEvent_t tr1) {
+  if (n_ == 0) return;
+  launch_cycle_begin(d_probs_, n_, max_bins_, s);
+  if (tr0) HIP_TRY(hipEventRecord(tr0, s));
+  if (woodcock_)
+    launch_woodcock(fp32_, lds_tally_, d_probs_, n_, grid_, lds_bytes_, tab_lds_, claim_, s);
+  else
+    launch_surface(d_probs_, n_, max_n_, s);
+  if (tr1) HIP_TRY(hipEventRecord(tr1, s));
+  int64_t groups = max_n_;
+  for (int l = 0; l < levels_; l++) {
+    groups = ceil_div(groups, kTreeWidth);
+    launch_ktree(d_probs_, n_, l, groups, s);
+  }
+  if (fission_any_) launch_bank(fp32_, d_probs_, n_, max_n_, max_tiles_, d_tile_sums_, d_tile_base_, s);
+  launch_finalize(d_probs_, d_fins_, n_, levels_, s);
+  HIP_TRY(hipGetLastError());
+}
+
+void Launcher::capture(hipStream_t s) {
+  drop_graph();
+  HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  enqueue_cycle(s, nullptr, nullptr);
+  HIP_TRY(hipStreamEndCapture(s, &graph_));
+  HIP_TRY(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+}
+
+void Launcher::launch_graph(hipStream_t s) { HIP_TRY(hipGraphLaunch(graph_exec_, s)); }
+
+// ---------------------------------------------------------------------------
+// Solve a batch to completion
+// ---------------------------------------------------------------------------
+namespace {
+struct StreamGuard {
+  hipStream_t s = nullptr;
+  std::vector<hipEvent_t> ev;
+  ~StreamGuard() {
+    for (auto e : ev) (void)hipEventDestroy(e);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  hipEvent_t make() {
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    ev.push_back(e);
+    return e;
+  }
+};
+}  // namespace
+
+int solve_batch(int n, const nraps_problem *const *pbs, const nraps_run_opts *opts,
+                nraps_result *const *results) {
+  auto t0 = std::chrono::steady_clock::now();
+  try {
+    if (n < 1 || !pbs || !opts) {
+      set_error("empty batch");
+      return NRAPS_E_INVALID;
+    }
+    for (int i = 0; i < n; i++) {
+      std::string why;
+      if (validate(pbs[i], &opts[i], &why)) {
+        set_error(n > 1 ? "problem " + std::to_string(i) + ": " + why : why);
+        return NRAPS_E_INVALID;
+      }
+      if (opts[i].precision != opts[0].precision || opts[i].method != opts[0].method) {
+        set_error("precision and method must agree across a batch");
+        return NRAPS_E_INVALID;
+      }
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+      set_error("no HIP device available");
+      return NRAPS_E_DEVICE;
+    }
+    const int device = opts[0].device;
+    if (device < 0 || device >= ndev) {
+      set_error("device ordinal out of range");
+      return NRAPS_E_DEVICE;
+    }
+    HIP_TRY(hipSetDevice(device));
+    const bool fp32 = opts[0].precision == NRAPS_FP32;
+    const bool woodcock = opts[0].method == NRAPS_EVENT_WOODCOCK;
+    const bool use_graph = opts[0].use_graph != 0;
+    std::vector<std::unique_ptr<ProblemDev>> owned(n);
+    std::vector<ProblemDev *> ps(n);
+    int gens_max = 0;
+    for (int i = 0; i < n; i++) {
+      owned[i].reset(new ProblemDev());
+      owned[i]->setup(pbs[i], opts[i], fp32, woodcock, nullptr, true, results ? results[i] : nullptr);
+      ps[i] = owned[i].get();
+      gens_max = std::max(gens_max, (int)opts[i].generations);
+    }
+    Launcher L(device);
+    L.build(ps, fp32, woodcock);
+    StreamGuard sg;
+    HIP_TRY(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
+    hipEvent_t e0 = sg.make(), e1 = sg.make();
+    std::vector<hipEvent_t> tr;
+    HIP_TRY(hipEventRecord(e0, sg.s));
+    if (use_graph) {
+      L.capture(sg.s);
+      for (int c = 0; c < gens_max; c++) L.launch_graph(sg.s);
+    } else {
+      for (int c = 0; c < 2 * gens_max; c++) tr.push_back(sg.make());
+      for (int c = 0; c < gens_max; c++) L.enqueue_cycle(sg.s, tr[2 * c], tr[2 * c + 1]);
+    }
+    HIP_TRY(hipEventRecord(e1, sg.s));
+    HIP_TRY(hipStreamSynchronize(sg.s));
+    float ms_dev = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms_dev, e0, e1));
+    double ms_tr = 0.0;
+    for (size_t c = 0; c + 1 < tr.size(); c += 2) {
+      float t = 0.f;
+      HIP_TRY(hipEventElapsedTime(&t, tr[c], tr[c + 1]));
+      ms_tr += t;
+    }
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    int worst = NRAPS_OK;
+    for (int i = 0; i < n; i++) {
+      int st = ps[i]->download(results ? results[i] : nullptr, ms, ms_dev, ms_tr,
+                               use_graph ? 0 : gens_max);
+      if (st != NRAPS_OK && worst == NRAPS_OK) worst = st;
+    }
+    return worst;
+  } catch (const DeviceError &e) {
+    set_error(e.what());
+    return e.err == hipErrorOutOfMemory ? NRAPS_E_NOMEM : NRAPS_E_DEVICE;
+  } catch (const std::invalid_argument &e) {
+    set_error(e.what());
+    return NRAPS_E_INVALID;
+  } catch (const std::bad_alloc &) {
+    set_error("host allocation failed");
+    return NRAPS_E_NOMEM;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return NRAPS_E_DEVICE;
+  }
+}
+
+}  // namespace nraps
+
+using namespace nraps;
+
+extern "C" int nraps_mc_solve(const nraps_problem *problem, const nraps_run_opts *opts,
+                              nraps_result *result) {
+  const nraps_problem *pbs[1] = {problem};
+  nraps_result *rs[1] = {result};
+  if (!opts) {
+    set_error("opts is NULL");
+    return NRAPS_E_INVALID;
+  }
+  int st = solve_batch(1, pbs, opts, rs);
+  if (result) result->status = st;
+  return st;
+}
+
+extern "C" int nraps_mc_solve_batch(int32_t n, const nraps_problem *const *problems,
+                                    const nraps_run_opts *opts, nraps_result *const *results) {
+  return solve_batch(n, problems, opts, results);
+}
+
+extern "C" int nraps_device_info(int32_t device, char *buf, int32_t buflen) {
+  if (!buf || buflen < 1) return NRAPS_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1 || device < 0 || device >= ndev) {
+    std::snprintf(buf, buflen, "nraps %d no-device", NRAPS_ABI_VERSION);
+    set_error("no HIP device available");
+    return NRAPS_E_DEVICE;
+  }
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, device) != hipSuccess) return NRAPS_E_DEVICE;
+  int lds = 0;
+  (void)hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+  std::snprintf(buf, buflen, "nraps %d %s cus=%d lds_per_block=%d hbm_bytes=%zu clock_khz=%d",
+                NRAPS_ABI_VERSION, p.gcnArchName, p.multiProcessorCount, lds, (size_t)p.totalGlobalMem,
+                p.clockRate);
+  return NRAPS_OK;
+}
+
+extern "C" int nraps_pcg32_fill(uint64_t seed, uint64_t stream0, int64_t n_streams, int32_t draws,
+                                uint32_t *out) {
+  try {
+    if (n_streams < 1 || draws < 1 || !out) {
+      set_error("bad arguments");
+      return NRAPS_E_INVALID;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+      set_error("no HIP device available");
+      return NRAPS_E_DEVICE;
+    }
+    DevArena a;
+    uint32_t *d = a.alloc<uint32_t>((size_t)n_streams * draws);
+    launch_pcg32_fill(seed, stream0, n_streams, draws, d, nullptr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, d, (size_t)n_streams * draws * 4, hipMemcpyDeviceToHost));
+    return NRAPS_OK;
+  } catch (const DeviceError &e) {
+    set_error(e.what());
+    return NRAPS_E_DEVICE;
+  }
+}

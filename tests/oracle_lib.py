@@ -1,0 +1,106 @@
+"""ctypes access to the CPU oracle (oracle/liboracle.so) — the checker."""
+import ctypes as C
+import os
+
+import numpy as np
+
+from nraps_amd import _abi
+from nraps_amd.problem import ResultBuffers
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_L = C.CDLL(os.path.join(ROOT, "oracle", "liboracle.so"))
+P, O, R = C.POINTER(_abi.Problem), C.POINTER(_abi.RunOpts), C.POINTER(_abi.Result)
+_dp = C.POINTER(C.c_double)
+_L.oracle_solve.argtypes = [P, O, C.c_int, R]
+_L.oracle_solve_banks.argtypes = [P, O, C.c_int, R, C.c_int32, _dp, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int64)]
+_L.oracle_hit_boundary.argtypes = [C.c_double] * 5 + [_dp]
+_L.oracle_cross_mesh.argtypes = [C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, _dp,
+                                 C.POINTER(C.c_int64)]
+_L.oracle_direction.argtypes = [C.c_double]
+_L.oracle_direction.restype = C.c_double
+_L.oracle_scat_mat_calc.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_double, _dp, _dp]
+_L.oracle_cdf_index.argtypes = [_dp, C.c_int32, C.c_double]
+_L.oracle_cdf_index.restype = C.c_int32
+_L.oracle_pcg32_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, C.POINTER(C.c_uint32)]
+_L.oracle_stream_id.argtypes = [C.c_uint64] * 3
+_L.oracle_stream_id.restype = C.c_uint64
+_L.oracle_canonical_sum.argtypes = [_dp, C.c_int64]
+_L.oracle_canonical_sum.restype = C.c_double
+_L.oracle_log.argtypes = [C.c_double]
+_L.oracle_log.restype = C.c_double
+_L.oracle_log_many.argtypes = [_dp, _dp, C.c_int64]
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def solve(problem, opts, threads=0, trace=False):
+    rb = ResultBuffers(problem, opts, trace=trace)
+    rb.status_code = _L.oracle_solve(C.byref(problem.view()), C.byref(opts), threads,
+                                     C.byref(rb.c))
+    return rb
+
+
+def solve_bank(problem, opts, cycle, threads=0):
+    N = int(opts.histories)
+    rb = ResultBuffers(problem, opts)
+    bx = np.zeros(N)
+    bc = np.zeros(N, dtype=np.int32)
+    bn = C.c_int64()
+    rb.status_code = _L.oracle_solve_banks(C.byref(problem.view()), C.byref(opts), threads,
+                                           C.byref(rb.c), cycle, _d(bx),
+                                           bc.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(bn))
+    return rb, bx[:bn.value], bc[:bn.value]
+
+
+def hit_boundary(mu, start_x, delta_s, bound, mesh_end):
+    out = np.zeros(3)
+    _L.oracle_hit_boundary(mu, start_x, delta_s, bound, mesh_end, _d(out))
+    return tuple(out)
+
+
+def cross_mesh(idx, mu, start_x, mesh_end, delta_s):
+    out = np.zeros(2)
+    ni = C.c_int64()
+    _L.oracle_cross_mesh(idx, mu, start_x, mesh_end, delta_s, _d(out), C.byref(ni))
+    return out[0], out[1], ni.value
+
+
+def direction(u):
+    return _L.oracle_direction(u)
+
+
+def scat_mat_calc(G, matid, g, inv_sigs, scat):
+    scat = np.ascontiguousarray(scat, dtype=np.float64)
+    out = np.zeros(G)
+    _L.oracle_scat_mat_calc(G, matid, g, inv_sigs, _d(scat), _d(out))
+    return out
+
+
+def cdf_index(cdf, u):
+    cdf = np.ascontiguousarray(cdf, dtype=np.float64)
+    return _L.oracle_cdf_index(_d(cdf), len(cdf), u)
+
+
+def pcg32(seed, stream, n):
+    out = np.zeros(n, dtype=np.uint32)
+    _L.oracle_pcg32_stream(seed, stream, n, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
+
+
+def stream_id(problem, cycle, p):
+    return _L.oracle_stream_id(problem, cycle, p)
+
+
+def canonical_sum(v):
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return _L.oracle_canonical_sum(_d(v), len(v))
+
+
+def log(x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.zeros_like(x)
+    _L.oracle_log_many(_d(x), _d(y), len(x))
+    return y
