@@ -92,7 +92,8 @@ NRAPS_HD uint32_t nraps_uindex(uint32_t u, uint32_t n) {
  * series coefficients 1/(2k+1).  ln 2 is split into a 42-bit head (e*hi is
  * exact for |e| < 2^11) and a tail, both rounded from the 60-digit value
  * 0.693147180559945309417232121458176568075500134360255254120680.
- * Error <= 1 ulp against a correctly rounded log (checked in the tests). */
+ * Error < 1.25 ulp against the exact logarithm (measured max 1.196 ulp over
+ * 30 000 samples against a 50-digit log; bound asserted in the tests). */
 #define NRAPS_LN2_HI 0x1.62e42fefa38p-1
 #define NRAPS_LN2_LO 0x1.ef35793c7673p-45
 #define NRAPS_SQRT2 0x1.6a09e667f3bcdp+0

@@ -139,7 +139,7 @@ typedef struct {
   const double *dx, *left, *right;
   int nf;
   int32_t *fuel;
-  double *inv_sigtr, *sigw, *abs_ratio, *inv_sigs, *nusigf, *chicdf, *cumrow;
+  double *inv_sigtr, *sigw, *abs_ratio, *inv_sigs, *nusigf, *chicdf;
   const double *scat;
   double maj;
   int R;
@@ -155,7 +155,6 @@ static void oprob_free(oprob *p) {
   free(p->inv_sigs);
   free(p->nusigf);
   free(p->chicdf);
-  free(p->cumrow);
   free(p->reg_of);
   free(p->reg_mat);
   free(p->reg_lo);
@@ -202,7 +201,6 @@ static int oprob_init(oprob *p, const nraps_problem *pb) {
   p->inv_sigs = malloc(sizeof(double) * GM);
   p->nusigf = malloc(sizeof(double) * GM);
   p->chicdf = malloc(sizeof(double) * GM);
-  p->cumrow = malloc(sizeof(double) * (size_t)M * G * G);
   for (int i = 0; i < GM; i++) {
     p->inv_sigtr[i] = pb->inv_sigtr ? pb->inv_sigtr[i]
                                     : 1.0 / (pb->sigt[i] - pb->mu[i] * pb->sigs[i]);
@@ -216,13 +214,6 @@ static int oprob_init(oprob *p, const nraps_problem *pb) {
     for (int g = 0; g < G; g++) {
       cum += pb->chit[m + M * g];
       p->chicdf[m * G + g] = cum;
-    }
-    for (int g = 0; g < G; g++) {
-      double c2 = 0.0;
-      for (int j = 0; j < G; j++) {
-        c2 += pb->scat_matrix[(size_t)G * G * m + (size_t)G * g + j];
-        p->cumrow[((size_t)m * G + g) * G + j] = c2;
-      }
     }
   }
   /* majorant over the materials that occur in the mesh */
@@ -373,8 +364,7 @@ static int ocollide(const ocycle *cy, owork *w, orng *r, int64_t pidx, int xs, i
     return 0;
   }
   double cdf[64];
-  const double *row = &p->cumrow[((size_t)m * G + *g) * G];
-  for (int j = 0; j < G; j++) cdf[j] = row[j] * p->inv_sigs[xs];
+  oracle_scat_mat_calc(G, m, *g, p->inv_sigs[xs], p->scat, cdf);
   int32_t gn = oracle_cdf_index(cdf, G, u_scat);
   otrace(w, NRAPS_EV_SCATTER, cell, *g, gn, x, *mu);
   *g = gn;
