@@ -19,7 +19,8 @@ __all__ = ["lib", "solve", "solve_batch", "generate", "load_deck", "write_output
            "ResultBuffers", "NrapsError"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libnraps.so")
+# NRAPS_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("NRAPS_LIB") or os.path.join(HERE, "libnraps.so")
 
 
 class NrapsError(RuntimeError):

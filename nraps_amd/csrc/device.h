@@ -49,6 +49,8 @@ struct DevProblem {
   int32_t nc, G, M, R, nf;
   int32_t xs_every, fission;
   int32_t tally_bins;  // 2*G*nc (direct + difference array)
+  int32_t nb;          // cell-lookup buckets
+  double inv_h;        // 1 / bucket width
   int64_t N;
   uint64_t seed, problem_id;
   int64_t max_events;

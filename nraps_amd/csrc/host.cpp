@@ -32,7 +32,7 @@ int validate(const nraps_problem *pb, const nraps_run_opts *o, std::string *why)
   };
   if (!pb) return bad("problem is NULL");
   const int nc = pb->n_cells, G = pb->n_groups, M = pb->n_mats;
-  if (nc < 1) return bad("n_cells must be >= 1");
+  if (nc < 1 || nc > kMaxCells) return bad("n_cells must be in [1, 65535]");
   if (G < 1 || G > kMaxGroups) return bad("n_groups must be in [1, 32]");
   if (M < 1 || M > kMaxMats) return bad("n_mats must be in [1, 255]");
   if (!pb->matid || !pb->delta_x || !pb->mesh_left || !pb->mesh_right)
@@ -199,6 +199,22 @@ void Prepared::build(const nraps_problem *pb) {
     }
   }
   nut_conv = pb->nut[0 + M * (G > 1 ? 1 : 0)];
+  // cell lookup buckets of width h (twice the narrowest cell, at least
+  // L / kMaxBuckets): bucket[b] = largest c with edges[c] < XL + b*h, so a
+  // point x is in cell bucket[floor((x - XL) / h)] or a few cells to its right
+  double hmin = XR - XL;
+  for (int c = 0; c < nc; c++) hmin = std::min(hmin, edges[c + 1] - edges[c]);
+  double h = std::max(2.0 * hmin, (XR - XL) / (kMaxBuckets - 2));
+  nb = (int)std::ceil((XR - XL) / h) + 2;
+  nb = std::min(nb, kMaxBuckets);
+  inv_h = 1.0 / h;
+  bucket.assign(nb, 0);
+  int c = 0;
+  for (int b = 0; b < nb; b++) {
+    double xb = XL + b * h;
+    while (c + 1 < nc && edges[c + 1] < xb) c++;
+    bucket[b] = (uint16_t)c;
+  }
 }
 
 TabLayout Prepared::layout(bool fp32) const {
@@ -212,9 +228,8 @@ TabLayout Prepared::layout(bool fp32) const {
   };
   const int GM = G * M;
   L.edges = put(rs * (nc + 1));
-  L.cellinfo = put(4 * nc);
-  L.fuel = put(4 * nf);
-  L.dx = put(rs * nc);
+  L.mat8 = put(nc);
+  L.reg16 = put(2 * nc);
   L.sigw = put(rs * GM);
   L.invtr = put(rs * GM);
   L.absr = put(rs * GM);
@@ -225,6 +240,10 @@ TabLayout Prepared::layout(bool fp32) const {
   L.nsfd = put(8 * GM);
   L.reglo = put(8 * R);
   L.kreg = put(8 * G * (R + 1));
+  L.bucket = put(2 * nb);
+  L.hot = at;
+  L.fuel = put(4 * nf);
+  L.dx = put(rs * nc);
   L.bytes = at;
   return L;
 }
@@ -247,8 +266,9 @@ std::vector<uint8_t> Prepared::pack(bool fp32, TabLayout *lay) const {
   };
   putr(L.edges, edges);
   for (int c = 0; c < nc; c++) {
-    uint32_t ci = (uint32_t)mat[c] | ((uint32_t)region[c] << 8);
-    std::memcpy(&blob[L.cellinfo + 4 * c], &ci, 4);
+    blob[L.mat8 + c] = (uint8_t)mat[c];
+    uint16_t r = (uint16_t)region[c];
+    std::memcpy(&blob[L.reg16 + 2 * c], &r, 2);
   }
   std::memcpy(&blob[L.fuel], fuel.data(), 4 * fuel.size());
   putr(L.dx, dx);
@@ -262,6 +282,7 @@ std::vector<uint8_t> Prepared::pack(bool fp32, TabLayout *lay) const {
   putd(L.nsfd, nusigf);
   putd(L.reglo, reg_lo);
   putd(L.kreg, kreg);
+  std::memcpy(&blob[L.bucket], bucket.data(), 2 * bucket.size());
   if (lay) *lay = L;
   return blob;
 }

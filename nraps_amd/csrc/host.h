@@ -23,12 +23,13 @@ constexpr int kMaxMats = 255;
 
 int validate(const nraps_problem *pb, const nraps_run_opts *o, std::string *why);
 
-// Byte offsets of each table inside the packed blob (all 8-byte aligned).
+// Byte offsets of each table inside the packed blob (all 16-byte aligned).
+// The first `hot` bytes are what the event kernel stages in LDS; the
+// birth-only tables after them stay in global memory.
 struct TabLayout {
   int32_t edges;    // Real[nc+1]   cell edges, edges[c] = mesh_left[c]
-  int32_t cellinfo; // u32[nc]      mat | region << 8
-  int32_t fuel;     // i32[nf]      source cells
-  int32_t dx;       // Real[nc]     Mesh.delta_x (birth position)
+  int32_t mat8;     // u8[nc]       material of each cell
+  int32_t reg16;    // u16[nc]      material region of each cell
   int32_t sigw;     // Real[G*M]    1/inv_sigtr (Woodcock site XS)
   int32_t invtr;    // Real[G*M]    inv_sigtr
   int32_t absr;     // Real[G*M]    siga/sigt
@@ -39,15 +40,23 @@ struct TabLayout {
   int32_t nsfd;     // double[G*M]  nut*sigf (k accumulation)
   int32_t reglo;    // double[R]    region left edges
   int32_t kreg;     // double[G*(R+1)] region prefix integrals of nu*Sigma_f
+  int32_t bucket;   // u16[nb]      cell lookup: largest c with edge[c] < XL + b*h
+  int32_t hot;      // bytes of the LDS-staged part (all of the above)
+  int32_t fuel;     // i32[nf]      source cells (birth only)
+  int32_t dx;       // Real[nc]     Mesh.delta_x (birth only)
   int32_t bytes;    // total (multiple of 16)
 };
 
+constexpr int kMaxBuckets = 8192;
+constexpr int kMaxCells = 65535;
+
 struct Prepared {
-  int nc = 0, G = 0, M = 0, R = 0, nf = 0, numass = 1;
-  double boundl = 0, boundr = 0, XL = 0, XR = 0, maj = 0, nut_conv = 0;
+  int nc = 0, G = 0, M = 0, R = 0, nf = 0, numass = 1, nb = 1;
+  double boundl = 0, boundr = 0, XL = 0, XR = 0, maj = 0, nut_conv = 0, inv_h = 0;
   std::vector<double> edges, dx, sigw, inv_sigtr, abs_ratio, inv_sigs, nusigf, chicdf,
       cumrow, reg_lo, kreg;
   std::vector<int32_t> mat, region, fuel, reg_mat;
+  std::vector<uint16_t> bucket;
 
   void build(const nraps_problem *pb);
   TabLayout layout(bool fp32) const;

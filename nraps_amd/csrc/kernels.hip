@@ -1,8 +1,8 @@
 // kernels.hip — HIP kernels of the MI355X Monte Carlo k-eigenvalue solver.
 //
 // Generation pipeline (every kernel takes the batch's DevProblem table and
-// works on any number of problems; blockIdx.y or the chunk list selects the
-// problem):
+// works on any number of problems; blockIdx.y or the block's problem walk
+// selects the problem):
 //
 //   k_cycle_begin      reset the history queues, zero the generation tallies
 //   k_surface          plain FP64 port: one thread per history, cell-by-cell
@@ -43,12 +43,14 @@ struct Num<double> {
   __device__ static double u01(uint32_t u) { return nraps_u01_f64(u); }
   __device__ static double flight(uint32_t u) { return nraps_uflight_f64(u); }
   __device__ static double nlog(double v) { return nraps_log(v); }
+  __device__ static double rcp(double v) { return 1.0 / v; }  // exact: CPU replay
 };
 template <>
 struct Num<float> {
   __device__ static float u01(uint32_t u) { return nraps_u01_f32(u); }
   __device__ static float flight(uint32_t u) { return nraps_uflight_f32(u); }
   __device__ static float nlog(float v) { return __logf(v); }
+  __device__ static float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 };
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
@@ -57,18 +59,22 @@ __device__ inline int rank_below(uint64_t mask) {
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Views of the packed tables (in LDS or in global memory).
+// Views of the packed tables: the hot part (in LDS or global memory) and the
+// birth-only part (global memory).
 template <typename Real>
 struct Tabs {
   const Real *edges, *dx, *sigw, *invtr, *absr, *invs, *nsf, *chicdf, *cumrow;
-  const uint32_t *cellinfo;
+  const uint8_t *mat8;
+  const uint16_t *reg16;
   const int32_t *fuel;
   const double *nsfd, *reglo, *kreg;
-  __device__ void bind(const uint8_t *base, const TabLayout &L) {
+  const uint16_t *bucket;
+  __device__ void bind(const uint8_t *base, const uint8_t *global_base, const TabLayout &L) {
     edges = (const Real *)(base + L.edges);
-    cellinfo = (const uint32_t *)(base + L.cellinfo);
-    fuel = (const int32_t *)(base + L.fuel);
-    dx = (const Real *)(base + L.dx);
+    mat8 = base + L.mat8;
+    reg16 = (const uint16_t *)(base + L.reg16);
+    fuel = (const int32_t *)(global_base + L.fuel);
+    dx = (const Real *)(global_base + L.dx);
     sigw = (const Real *)(base + L.sigw);
     invtr = (const Real *)(base + L.invtr);
     absr = (const Real *)(base + L.absr);
@@ -79,9 +85,80 @@ struct Tabs {
     nsfd = (const double *)(base + L.nsfd);
     reglo = (const double *)(base + L.reglo);
     kreg = (const double *)(base + L.kreg);
+    bucket = (const uint16_t *)(base + L.bucket);
   }
-  __device__ int mat(int c) const { return (int)(cellinfo[c] & 0xffu); }
-  __device__ int reg(int c) const { return (int)(cellinfo[c] >> 8); }
+  __device__ int mat(int c) const { return (int)mat8[c]; }
+  __device__ int reg(int c) const { return (int)reg16[c]; }
+};
+
+// Uniform values of one problem, read once per problem visit so the hot
+// loop keeps them in scalar registers.
+template <typename Real>
+struct Ctx {
+  Tabs<Real> T;
+  int nc, G, M, R, nf, nb;
+  int xs_every, fission, from_bank, make_sites, cycle;
+  int64_t max_events, N;
+  uint64_t seed, problem_id;
+  Real maj, inv_maj, XL, XR, bl, br, inv_k, inv_h;
+  double *kacc;
+  int32_t *counts;
+  const Real *src_x;
+  const int32_t *src_cell;
+  Real *site_x;
+  int32_t *site_cell;
+  uint64_t *site_key;
+  unsigned long long *site_count;
+  int64_t site_cap;
+  nraps_trace_event *trace;
+  int32_t *trace_len;
+  int64_t *draws;
+  double *trace_kacc;
+  int64_t trace_n;
+  int trace_ev, trace_cycle;
+
+  __device__ void load(const DevProblem &P, const CycleState &cs, const uint8_t *tab) {
+    T.bind(tab, P.tab, P.lay);
+    nc = P.nc;
+    G = P.G;
+    M = P.M;
+    R = P.R;
+    nf = P.nf;
+    nb = P.nb;
+    xs_every = P.xs_every;
+    fission = P.fission;
+    from_bank = cs.from_bank;
+    make_sites = cs.make_sites;
+    cycle = cs.cycle;
+    max_events = P.max_events;
+    N = P.N;
+    seed = P.seed;
+    problem_id = P.problem_id;
+    maj = (Real)P.maj;
+    inv_maj = (Real)P.inv_maj;
+    XL = (Real)P.XL;
+    XR = (Real)P.XR;
+    bl = (Real)P.boundl;
+    br = (Real)P.boundr;
+    inv_k = (Real)cs.inv_k;
+    inv_h = (Real)P.inv_h;
+    kacc = P.kacc;
+    counts = P.counts;
+    src_x = (const Real *)P.src_x;
+    src_cell = P.src_cell;
+    site_x = (Real *)P.site_x;
+    site_cell = P.site_cell;
+    site_key = P.site_key;
+    site_count = &P.cs->site_count;
+    site_cap = P.site_cap;
+    trace = P.trace;
+    trace_len = P.trace_len;
+    draws = P.draws;
+    trace_kacc = P.trace_kacc;
+    trace_n = P.trace_n;
+    trace_ev = P.trace_ev;
+    trace_cycle = P.trace_cycle;
+  }
 };
 
 struct LaneRng {
@@ -97,12 +174,33 @@ struct LocalStats {
   uint32_t v[kNumStats];
 };
 
+// Compile-time features of a history kernel: per-history parity traces and
+// fission-site banking.  Production launches without them carry neither the
+// code nor the registers.
+template <bool TRACE_, bool FISSION_>
+struct Feat {
+  static constexpr bool TRACE = TRACE_;
+  static constexpr bool FISSION = FISSION_;
+};
+using FeatAll = Feat<true, true>;
+
+// Outcome bits of one Woodcock step.
+enum : uint32_t {
+  kEvStep = 1u,     // a flight was sampled
+  kEvReal = 2u,     // real collision
+  kEvAbsorb = 4u,   // absorbed
+  kEvLeak = 8u,     // leaked through a vacuum face
+  kEvCap = 16u,     // stopped by the event cap
+  kEvDead = kEvAbsorb | kEvLeak | kEvCap,
+};
+
 template <typename Real>
 struct Particle {
   Real x, mu;
   int32_t cell, g, gxs, rxs;
-  int64_t p, events;
+  int32_t p, events;
   double kacc;
+  double kcur;  // k integral at (x, cell, g), carried between Woodcock steps
   int32_t nsites, tr;
   bool traced;
 };
@@ -116,11 +214,11 @@ __device__ inline int cdf_index(const Real *cdf, int n, Real u) {
   return i < n - 1 ? i : n - 1;
 }
 
-template <typename Real>
-__device__ inline void trace_event(const DevProblem &P, Particle<Real> &q, int type, int cell, int g,
+template <class F, typename Real>
+__device__ inline void trace_event(const Ctx<Real> &C, Particle<Real> &q, int type, int cell, int g,
                                    int aux, Real x, Real mu) {
-  if (!q.traced) return;
-  if (q.tr < P.trace_ev) {
+  if (!F::TRACE || !q.traced) return;
+  if (q.tr < C.trace_ev) {
     nraps_trace_event e;
     e.type = type;
     e.cell = cell;
@@ -128,20 +226,32 @@ __device__ inline void trace_event(const DevProblem &P, Particle<Real> &q, int t
     e.aux = aux;
     e.x = (double)x;
     e.mu = (double)mu;
-    P.trace[q.p * P.trace_ev + q.tr] = e;
+    C.trace[(int64_t)q.p * C.trace_ev + q.tr] = e;
   }
   q.tr++;
+}
+
+template <class F, typename Real>
+__device__ inline void init_particle(const Ctx<Real> &C, int64_t p, LaneRng &rng, Particle<Real> &q) {
+  nraps_history_rng(&rng.r, C.seed, nraps_stream_id(C.problem_id, (uint64_t)C.cycle, (uint64_t)p));
+  rng.draws = 0;
+  q.p = (int32_t)p;
+  q.events = 0;
+  q.kacc = 0.0;
+  q.nsites = 0;
+  q.tr = 0;
+  q.traced = F::TRACE && C.trace != nullptr && C.cycle == C.trace_cycle && p < C.trace_n;
 }
 
 // Birth, src/mc_code.rs:39-53 and :228-230 (uniform fuel cell, position,
 // direction, chi group), or a banked fission site (direction, chi group).
 template <typename Real>
-__device__ inline void birth(const DevProblem &P, const Tabs<Real> &T, bool from_bank, int64_t p,
-                             LaneRng &rng, Particle<Real> &q) {
-  const int G = P.G;
-  if (!from_bank) {
+__device__ inline void birth(const Ctx<Real> &C, int64_t p, LaneRng &rng, Particle<Real> &q) {
+  const Tabs<Real> &T = C.T;
+  const int G = C.G;
+  if (!C.from_bank) {
     uint32_t u = rng.next();
-    int c = T.fuel[nraps_uindex(u, (uint32_t)P.nf)];
+    int c = T.fuel[nraps_uindex(u, (uint32_t)C.nf)];
     Real frac = Num<Real>::u01(rng.next());
     q.mu = Real(2) * Num<Real>::u01(rng.next()) - Real(1);
     Real ue = Num<Real>::u01(rng.next());
@@ -152,80 +262,94 @@ __device__ inline void birth(const DevProblem &P, const Tabs<Real> &T, bool from
   } else {
     q.mu = Real(2) * Num<Real>::u01(rng.next()) - Real(1);
     Real ue = Num<Real>::u01(rng.next());
-    int c = P.src_cell[p];
+    int c = C.src_cell[p];
     q.cell = c;
-    q.x = ((const Real *)P.src_x)[p];
+    q.x = C.src_x[p];
     q.g = cdf_index(&T.chicdf[T.mat(c) * G], G, ue);
   }
 }
 
-// Fission sites of one collision.  WAVE: append by ballot + one atomic per
-// wave (event pipeline); otherwise one atomic per site (plain port).
-template <typename Real, bool WAVE>
-__device__ inline void emit_sites(const DevProblem &P, int n, Real x, int cell, Particle<Real> &q) {
-  if (WAVE) {
-    int rem = n;
-    for (;;) {
-      bool want = rem > 0;
-      uint64_t m = __ballot(want);
-      if (m == 0) break;
-      int cnt = __popcll(m);
-      int leader = __ffsll((unsigned long long)m) - 1;
-      unsigned long long base = 0;
-      if (lane_id() == leader) base = atomicAdd(&P.cs->site_count, (unsigned long long)cnt);
-      base = __shfl(base, leader);
-      if (want) {
-        unsigned long long idx = base + (unsigned long long)rank_below(m);
-        if (idx < (unsigned long long)P.site_cap) {
-          ((Real *)P.site_x)[idx] = x;
-          P.site_cell[idx] = cell;
-          P.site_key[idx] = ((uint64_t)q.p << 32) | (uint32_t)q.nsites;
-        }
-        q.nsites++;
-        rem--;
-      }
-    }
-  } else {
-    for (int i = 0; i < n; i++) {
-      unsigned long long idx = atomicAdd(&P.cs->site_count, 1ull);
-      if (idx < (unsigned long long)P.site_cap) {
-        ((Real *)P.site_x)[idx] = x;
-        P.site_cell[idx] = cell;
-        P.site_key[idx] = ((uint64_t)q.p << 32) | (uint32_t)q.nsites;
-      }
-      q.nsites++;
-    }
+template <typename Real>
+__device__ inline void put_site(const Ctx<Real> &C, unsigned long long idx, Particle<Real> &q) {
+  if (idx < (unsigned long long)C.site_cap) {
+    C.site_x[idx] = q.x;
+    C.site_cell[idx] = q.cell;
+    C.site_key[idx] = ((uint64_t)(uint32_t)q.p << 32) | (uint32_t)q.nsites;
   }
+  q.nsites++;
 }
 
-// Collision, src/mc_code.rs:113-132 (draw order U_int, U_mu, U_scat; the
-// scatter CDF uses the row of the current group and the normaliser of the
-// possibly stale XS index), then fission sites when banking.  Returns true
-// if the particle scattered.
-template <typename Real, bool WAVE>
-__device__ inline bool collide(const DevProblem &P, const Tabs<Real> &T, const CycleState &cs,
-                               Real inv_k, int xs, LaneRng &rng, Particle<Real> &q, LocalStats &st) {
-  const int G = P.G, M = P.M, m = T.mat(q.cell);
+// Fission sites of the plain port: one atomic per site.
+template <typename Real>
+__device__ inline void emit_sites_plain(const Ctx<Real> &C, int n, Particle<Real> &q) {
+  for (int i = 0; i < n; i++) put_site(C, atomicAdd(C.site_count, 1ull), q);
+}
+
+// Site-slot cursor of one wave in the event pipeline.  Slots of the
+// unordered site buffer are reserved kSiteChunk at a time with one atomic;
+// lanes fill them by ballot + mbcnt.  The unused tail of a chunk is marked
+// with an invalid key, which the placement kernel skips.  Must be used in
+// wave-uniform control flow (every lane holds the same cursor).
+constexpr int kSiteChunk = 128;
+constexpr uint64_t kNoSite = ~0ull;
+
+struct SiteCursor {
+  unsigned long long next = 0, end = 0;
+  template <typename Real>
+  __device__ void close(const Ctx<Real> &C) {
+    for (unsigned long long i = next + (unsigned long long)lane_id(); i < end; i += 64)
+      if (i < (unsigned long long)C.site_cap) C.site_key[i] = kNoSite;
+    next = end;
+  }
+  // Bank `n` sites of this lane's particle at its position (n may be 0).
+  template <typename Real>
+  __device__ void emit(const Ctx<Real> &C, int n, Particle<Real> &q) {
+    int rem = n;
+    for (;;) {
+      const bool want = rem > 0;
+      const uint64_t m = __ballot(want);
+      if (m == 0) break;
+      const unsigned long long cnt = (unsigned long long)__popcll(m);
+      if (next + cnt > end) {
+        close(C);
+        unsigned long long b = 0;
+        if (lane_id() == 0) b = atomicAdd(C.site_count, (unsigned long long)kSiteChunk);
+        next = __shfl(b, 0);
+        end = next + kSiteChunk;
+      }
+      if (want) {
+        put_site(C, next + (unsigned long long)rank_below(m), q);
+        rem--;
+      }
+      next += cnt;
+    }
+  }
+};
+
+// Collision at (q.x, q.cell), src/mc_code.rs:113-132: draws U_int, U_mu,
+// U_scat; the scatter CDF uses the row of the current group and the
+// normaliser of the (possibly stale) XS index; the number of fission sites
+// is drawn after the reference's three draws and returned in `nsite` (the
+// caller banks them at the collision site).  Returns true if the particle
+// scattered.
+template <class F, typename Real>
+__device__ inline bool collide(const Ctx<Real> &C, int xs, LaneRng &rng, Particle<Real> &q,
+                               int &nsite) {
+  const Tabs<Real> &T = C.T;
+  const int G = C.G, m = T.mat(q.cell);
   Real u_int = Num<Real>::u01(rng.next());
   Real mu_new = Real(2) * Num<Real>::u01(rng.next()) - Real(1);
   Real u_scat = Num<Real>::u01(rng.next());
-  int nsite = 0;
-  if (cs.make_sites) {
-    Real nsf = T.nsf[m + M * q.g];
+  nsite = 0;
+  if (F::FISSION && C.make_sites) {
+    Real nsf = T.nsf[m + C.M * q.g];
     if (nsf > Real(0)) {
-      Real y = nsf * T.invtr[xs] * inv_k + Num<Real>::u01(rng.next());
+      Real y = nsf * T.invtr[xs] * C.inv_k + Num<Real>::u01(rng.next());
       nsite = y < Real(kMaxSitesPerCollision) ? (int)y : kMaxSitesPerCollision;
     }
   }
-  if (WAVE) {
-    if (cs.make_sites) emit_sites<Real, true>(P, nsite, q.x, q.cell, q);
-  } else if (nsite) {
-    emit_sites<Real, false>(P, nsite, q.x, q.cell, q);
-  }
-  st.v[kStCollisions]++;
   if (u_int < T.absr[xs]) {
-    st.v[kStAbsorb]++;
-    trace_event(P, q, NRAPS_EV_ABSORB, q.cell, q.g, q.g, q.x, q.mu);
+    trace_event<F>(C, q, NRAPS_EV_ABSORB, q.cell, q.g, q.g, q.x, q.mu);
     return false;
   }
   const Real *row = &T.cumrow[(m * G + q.g) * G];
@@ -237,35 +361,21 @@ __device__ inline bool collide(const DevProblem &P, const Tabs<Real> &T, const C
       break;
     }
   }
-  trace_event(P, q, NRAPS_EV_SCATTER, q.cell, q.g, gn, q.x, q.mu);
+  trace_event<F>(C, q, NRAPS_EV_SCATTER, q.cell, q.g, gn, q.x, q.mu);
   q.g = gn;
   q.mu = mu_new;
   return true;
 }
 
-template <typename Real>
-__device__ inline void finish(const DevProblem &P, const CycleState &cs, Particle<Real> &q,
-                              const LaneRng &rng) {
-  P.kacc[q.p] = q.kacc;
-  if (P.fission) P.counts[q.p] = q.nsites;
-  if (q.traced) {
-    P.trace_len[q.p] = q.tr;
-    P.draws[q.p] = rng.draws;
-    P.trace_kacc[q.p] = q.kacc;
+template <class F, typename Real>
+__device__ inline void finish(const Ctx<Real> &C, Particle<Real> &q, const LaneRng &rng) {
+  C.kacc[q.p] = q.kacc;
+  if (F::FISSION && C.fission) C.counts[q.p] = q.nsites;
+  if (F::TRACE && q.traced) {
+    C.trace_len[q.p] = q.tr;
+    C.draws[q.p] = rng.draws;
+    C.trace_kacc[q.p] = q.kacc;
   }
-}
-
-template <typename Real>
-__device__ inline void init_particle(const DevProblem &P, const CycleState &cs, int64_t p,
-                                     LaneRng &rng, Particle<Real> &q) {
-  nraps_pcg32_seed(&rng.r, P.seed, nraps_stream_id(P.problem_id, (uint64_t)cs.cycle, (uint64_t)p));
-  rng.draws = 0;
-  q.p = p;
-  q.events = 0;
-  q.kacc = 0.0;
-  q.nsites = 0;
-  q.tr = 0;
-  q.traced = P.trace != nullptr && cs.cycle == P.trace_cycle && p < P.trace_n;
 }
 
 __device__ inline void flush_stats(const DevProblem &P, LocalStats &st) {
@@ -297,26 +407,25 @@ __device__ inline double seg_len(double a, double b, double mu) {
   return a == b ? 0.0 : fabs((a - b) / mu);
 }
 
-__device__ void history_surface(const DevProblem &P, const Tabs<double> &T, const CycleState &cs,
-                                int64_t p, LocalStats &st) {
-  const int nc = P.nc, M = P.M;
-  const bool xs_every = P.xs_every;
+__device__ void history_surface(const Ctx<double> &C, const DevProblem &P, int64_t p, LocalStats &st) {
+  const Tabs<double> &T = C.T;
+  const int nc = C.nc, M = C.M;
+  const bool xs_every = C.xs_every;
   double *tally = P.tally_glob;
   LaneRng rng;
   Particle<double> q;
-  init_particle(P, cs, p, rng, q);
-  birth(P, T, cs.from_bank, p, rng, q);
-  trace_event(P, q, NRAPS_EV_BIRTH, q.cell, q.g, 0, q.x, q.mu);
-  const double inv_k = cs.inv_k;
+  init_particle<FeatAll>(C, p, rng, q);
+  birth(C, p, rng, q);
+  trace_event<FeatAll>(C, q, NRAPS_EV_BIRTH, q.cell, q.g, 0, q.x, q.mu);
   for (;;) {
     int xs = T.mat(q.cell) + M * q.g;
     double delta_s = q.mu * -nraps_log(nraps_uflight_f64(rng.next())) * T.invtr[xs];
     bool same = true;
     while (same) {
-      if (++q.events > P.max_events) {
+      if (++q.events > C.max_events) {
         st.v[kStCapped]++;
-        trace_event(P, q, NRAPS_EV_CAP, q.cell, q.g, 0, q.x, q.mu);
-        finish(P, cs, q, rng);
+        trace_event<FeatAll>(C, q, NRAPS_EV_CAP, q.cell, q.g, 0, q.x, q.mu);
+        finish<FeatAll>(C, q, rng);
         return;
       }
       const int cell = q.cell;
@@ -337,8 +446,8 @@ __device__ void history_surface(const DevProblem &P, const Tabs<double> &T, cons
           q.x = mesh_end;
         } else {
           st.v[kStLeak]++;
-          trace_event(P, q, NRAPS_EV_LEAK, cell, q.g, 0, mesh_end, mu);
-          finish(P, cs, q, rng);
+          trace_event<FeatAll>(C, q, NRAPS_EV_LEAK, cell, q.g, 0, mesh_end, mu);
+          finish<FeatAll>(C, q, rng);
           return;
         }
       } else if (fabs(end_x - x) > fabs(mesh_end - x)) {
@@ -356,8 +465,13 @@ __device__ void history_surface(const DevProblem &P, const Tabs<double> &T, cons
         atomicAdd(&tally[q.g * nc + cell], tl);
         q.kacc += T.nsfd[mc + M * q.g] * tl;
         q.x = end_x;
-        if (!collide<double, false>(P, T, cs, inv_k, xs, rng, q, st)) {
-          finish(P, cs, q, rng);
+        st.v[kStCollisions]++;
+        int nsite;
+        const bool scattered = collide<FeatAll, double>(C, xs, rng, q, nsite);
+        if (nsite) emit_sites_plain(C, nsite, q);
+        if (!scattered) {
+          st.v[kStAbsorb]++;
+          finish<FeatAll>(C, q, rng);
           return;
         }
         if (xs_every) xs = T.mat(q.cell) + M * q.g;
@@ -373,11 +487,11 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevProblem *probs, int
   LocalStats st;
   for (int i = 0; i < kNumStats; i++) st.v[i] = 0;
   if (cs.active) {
-    Tabs<double> T;
-    T.bind(P.tab, P.lay);
-    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < P.N;
+    Ctx<double> C;
+    C.load(P, cs, P.tab);
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < C.N;
          p += (int64_t)gridDim.x * blockDim.x)
-      history_surface(P, T, cs, p, st);
+      history_surface(C, P, p, st);
   }
   flush_stats(P, st);
 }
@@ -387,18 +501,17 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevProblem *probs, int
 // ---------------------------------------------------------------------------
 // Cell holding the end of a move: moving right (mu >= 0) a point on an edge
 // belongs to the cell on its left, moving left to the cell on its right
-// (the strict crossing test of src/mc_code.rs:171).  Fixed-step binary
-// search over the cell edges.
+// (the strict crossing test of src/mc_code.rs:171).  The bucket table gives
+// a starting cell; short walks make the answer exact whatever the rounding
+// of the bucket index.
 template <typename Real>
-__device__ inline int locate(const Real *edges, int nc, int top, Real xe, bool right) {
-  int c = 0;
-  for (int step = top; step > 0; step >>= 1) {
-    int t = c + step;
-    if (t < nc) {
-      Real e = edges[t];
-      if (right ? (e < xe) : (e <= xe)) c = t;
-    }
-  }
+__device__ inline int locate(const Ctx<Real> &C, Real xe, bool right) {
+  const Real *e = C.T.edges;
+  int b = (int)((xe - C.XL) * C.inv_h);
+  b = b < 0 ? 0 : (b >= C.nb ? C.nb - 1 : b);
+  int c = C.T.bucket[b];
+  while (c > 0 && !(right ? e[c] < xe : e[c] <= xe)) c--;
+  while (c + 1 < C.nc && (right ? e[c + 1] < xe : e[c + 1] <= xe)) c++;
   return c;
 }
 
@@ -432,111 +545,153 @@ __device__ inline void tally_piece(Acc *direct, Acc *diff, const Real *edges, in
 // Integral of nu*Sigma_f from the slab's left edge to x (group g), in FP64:
 // the k contribution of a piece is the difference at its ends times 1/|mu|.
 template <typename Real>
-__device__ inline double kint(const Tabs<Real> &T, int M, int R, int g, Real x, int c) {
+__device__ inline double kint(const Ctx<Real> &C, int g, Real x, int c) {
+  const Tabs<Real> &T = C.T;
   int r = T.reg(c);
-  return T.kreg[g * (R + 1) + r] + T.nsfd[T.mat(c) + M * g] * ((double)x - T.reglo[r]);
+  return T.kreg[g * (C.R + 1) + r] + T.nsfd[T.mat(c) + C.M * g] * ((double)x - T.reglo[r]);
 }
 
 // One Woodcock step: flight to a tentative site (folding reflections),
 // track-length tally, XS refresh when the path left the region of the last
-// refresh, acceptance test, and the collision if real.  Returns false when
-// the particle leaks, is absorbed or hits the event cap.
-template <typename Real, typename Acc>
-__device__ inline bool woodcock_step(const DevProblem &P, const Tabs<Real> &T, const CycleState &cs,
-                                     Real inv_k, Real maj, Real inv_maj, Real XL, Real XR,
-                                     Real bl, Real br, int top, Acc *direct, Acc *diff,
-                                     LaneRng &rng, Particle<Real> &q, LocalStats &st) {
-  const int nc = P.nc, M = P.M, R = P.R;
-  if (++q.events > P.max_events) {
-    st.v[kStCapped]++;
-    trace_event(P, q, NRAPS_EV_CAP, q.cell, q.g, 0, q.x, q.mu);
-    return false;
+// refresh, acceptance test, and the collision if real.  Returns kEv* bits;
+// boundary crossings are added to `ncross`, fission sites to bank at the
+// particle's position are returned in `nsite`.
+template <class F, typename Real, typename Acc>
+__device__ inline uint32_t woodcock_step(const Ctx<Real> &C, Acc *direct, Acc *diff, LaneRng &rng,
+                                         Particle<Real> &q, uint32_t &ncross, int &nsite) {
+  const Tabs<Real> &T = C.T;
+  const int nc = C.nc;
+  if (++q.events > C.max_events) {
+    trace_event<F>(C, q, NRAPS_EV_CAP, q.cell, q.g, 0, q.x, q.mu);
+    return kEvCap;
   }
-  st.v[kStSteps]++;
   Real mu = q.mu, x = q.x;
   int cell = q.cell;
   const int g = q.g, gbase = g * nc;
-  Real disp = mu * -Num<Real>::nlog(Num<Real>::flight(rng.next())) * inv_maj;
+  Real disp = mu * -Num<Real>::nlog(Num<Real>::flight(rng.next())) * C.inv_maj;
   bool left_region = false;
-  Real wgt = Real(1) / fabs(mu);
+  Real wgt = Num<Real>::rcp(fabs(mu));
+  double kc = q.kcur;
   for (;;) {
     Real end = x + disp;
-    if (mu >= Real(0) && end > XR) {
-      tally_piece<Real, Acc>(direct, diff, T.edges, gbase, x, cell, XR, nc - 1, wgt);
-      if (x != XR) q.kacc += fabs(kint(T, M, R, g, XR, nc - 1) - kint(T, M, R, g, x, cell)) * (double)wgt;
+    if (mu >= Real(0) && end > C.XR) {
+      tally_piece<Real, Acc>(direct, diff, T.edges, gbase, x, cell, C.XR, nc - 1, wgt);
+      double kb = kint(C, g, C.XR, nc - 1);
+      if (x != C.XR) q.kacc += fabs(kb - kc) * (double)wgt;
+      kc = kb;
       if (T.reg(nc - 1) != q.rxs) left_region = true;
-      st.v[kStCrossings]++;
-      if (br > Real(0)) {
-        Real m2 = mu * (-br);
-        disp = (disp + (x - XR)) * (-br);
-        x = XR;
+      ncross++;
+      if (C.br > Real(0)) {
+        Real m2 = mu * (-C.br);
+        disp = (disp + (x - C.XR)) * (-C.br);
+        x = C.XR;
         mu = m2;
         cell = nc - 1;
-        wgt = Real(1) / fabs(mu);
+        wgt = Num<Real>::rcp(fabs(mu));
         continue;
       }
-      st.v[kStLeak]++;
-      trace_event(P, q, NRAPS_EV_LEAK, nc - 1, g, 0, XR, mu);
-      return false;
-    } else if (mu < Real(0) && XL > end) {
-      tally_piece<Real, Acc>(direct, diff, T.edges, gbase, x, cell, XL, 0, wgt);
-      if (x != XL) q.kacc += fabs(kint(T, M, R, g, XL, 0) - kint(T, M, R, g, x, cell)) * (double)wgt;
+      trace_event<F>(C, q, NRAPS_EV_LEAK, nc - 1, g, 0, C.XR, mu);
+      return kEvStep | kEvLeak;
+    } else if (mu < Real(0) && C.XL > end) {
+      tally_piece<Real, Acc>(direct, diff, T.edges, gbase, x, cell, C.XL, 0, wgt);
+      double kb = kint(C, g, C.XL, 0);
+      if (x != C.XL) q.kacc += fabs(kb - kc) * (double)wgt;
+      kc = kb;
       if (T.reg(0) != q.rxs) left_region = true;
-      st.v[kStCrossings]++;
-      if (bl > Real(0)) {
-        Real m2 = mu * (-bl);
-        disp = (disp + (x - XL)) * (-bl);
-        x = XL;
+      ncross++;
+      if (C.bl > Real(0)) {
+        Real m2 = mu * (-C.bl);
+        disp = (disp + (x - C.XL)) * (-C.bl);
+        x = C.XL;
         mu = m2;
         cell = 0;
-        wgt = Real(1) / fabs(mu);
+        wgt = Num<Real>::rcp(fabs(mu));
         continue;
       }
-      st.v[kStLeak]++;
-      trace_event(P, q, NRAPS_EV_LEAK, 0, g, 0, XL, mu);
-      return false;
+      trace_event<F>(C, q, NRAPS_EV_LEAK, 0, g, 0, C.XL, mu);
+      return kEvStep | kEvLeak;
     } else {
-      int ce = end == x ? cell : locate(T.edges, nc, top, end, mu >= Real(0));
-      tally_piece<Real, Acc>(direct, diff, T.edges, gbase, x, cell, end, ce, wgt);
-      if (end != x) q.kacc += fabs(kint(T, M, R, g, end, ce) - kint(T, M, R, g, x, cell)) * (double)wgt;
-      x = end;
-      cell = ce;
+      if (end != x) {
+        int ce = locate(C, end, mu >= Real(0));
+        tally_piece<Real, Acc>(direct, diff, T.edges, gbase, x, cell, end, ce, wgt);
+        double kb = kint(C, g, end, ce);
+        q.kacc += fabs(kb - kc) * (double)wgt;
+        kc = kb;
+        x = end;
+        cell = ce;
+      }
       break;
     }
   }
   q.x = x;
   q.mu = mu;
   q.cell = cell;
-  if (!P.xs_every && (left_region || T.reg(cell) != q.rxs)) {
+  q.kcur = kc;
+  if (!C.xs_every && (left_region || T.reg(cell) != q.rxs)) {
     q.gxs = g;
     q.rxs = T.reg(cell);
   }
-  const int xs = T.mat(cell) + M * q.gxs;
+  const int xs = T.mat(cell) + C.M * q.gxs;
   Real u_acc = Num<Real>::u01(rng.next());
-  if (u_acc * maj < T.sigw[xs]) {
-    if (!collide<Real, true>(P, T, cs, inv_k, xs, rng, q, st)) return false;
-    if (P.xs_every) {
-      q.gxs = q.g;
-      q.rxs = T.reg(cell);
-    }
-  } else {
-    st.v[kStVirtual]++;
+  if (!(u_acc * C.maj < T.sigw[xs])) return kEvStep;
+  if (!collide<F, Real>(C, xs, rng, q, nsite)) return kEvStep | kEvReal | kEvAbsorb;
+  if (q.g != g) q.kcur = kint(C, q.g, q.x, q.cell);
+  if (C.xs_every) {
+    q.gxs = q.g;
+    q.rxs = T.reg(cell);
   }
-  return true;
+  return kEvStep | kEvReal;
 }
+
+// Per-wave counters in scalar registers: every update is the population
+// count of a ballot taken in wave-uniform control flow.
+struct WaveStats {
+  uint64_t steps = 0, real = 0, absorb = 0, leak = 0, cap = 0;
+  __device__ void add(uint32_t ev) {
+    steps += __popcll(__ballot(ev & kEvStep));
+    real += __popcll(__ballot(ev & kEvReal));
+    absorb += __popcll(__ballot(ev & kEvAbsorb));
+    leak += __popcll(__ballot(ev & kEvLeak));
+    cap += __popcll(__ballot(ev & kEvCap));
+  }
+  __device__ void flush(const DevProblem &P, uint32_t ncross) {
+    unsigned long long x = ncross;
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
+    if (lane_id() == 0) {
+      if (real) atomicAdd(&P.stats[kStCollisions], (unsigned long long)real);
+      if (steps) atomicAdd(&P.stats[kStSteps], (unsigned long long)steps);
+      if (absorb) atomicAdd(&P.stats[kStAbsorb], (unsigned long long)absorb);
+      if (leak) atomicAdd(&P.stats[kStLeak], (unsigned long long)leak);
+      if (cap) atomicAdd(&P.stats[kStCapped], (unsigned long long)cap);
+      if (x) atomicAdd(&P.stats[kStCrossings], x);
+      // virtual collisions: flights that were neither real nor a leak
+      const uint64_t virt = steps - real - leak;
+      if (virt) atomicAdd(&P.stats[kStVirtual], (unsigned long long)virt);
+    }
+  }
+};
+
+// Minimum resident waves per SIMD requested from the register allocator.
+#ifndef NRAPS_WC_WAVES_PER_EU
+#define NRAPS_WC_WAVES_PER_EU 3
+#endif
 
 // Persistent event kernel.  A block visits the problems of the batch in
 // cyclic order starting at its own index; on a problem with histories left
 // it stages the tables in LDS, zeroes its LDS tally, and its waves run
 // histories until the problem's queue is empty: each wave claims `claim`
 // histories with one global atomic on the queue head and refills idle lanes
-// from that range by ballot + mbcnt, so lanes stay busy without block
-// barriers.  The LDS tally is then flushed with FP64 atomics.  Tallies that
-// do not fit in LDS are scored straight into the global FP64 tally.
-// Dynamic LDS: [tables][tally direct|diff].
-template <typename Real, bool LDS_TALLY>
-__global__ __launch_bounds__(kBlock) void k_woodcock(const DevProblem *probs, int nprob,
-                                                     int tab_lds_bytes, int claim) {
+// from that range by ballot + mbcnt once at least `refill_min` lanes are
+// idle (or none is busy), so lanes stay busy without block barriers.  The
+// LDS tally is then flushed with FP64 atomics.  Tallies that do not fit in
+// LDS are scored straight into the global FP64 tally.  BS is the block
+// size: 256 for small LDS footprints (several blocks per CU), 1024 when one
+// block per CU uses the whole LDS.
+// Dynamic LDS: [hot tables][tally direct|diff].
+template <typename Real, bool LDS_TALLY, bool TRACE, bool FISSION, int BS>
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(NRAPS_WC_WAVES_PER_EU)))
+void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim, int refill_min) {
+  using F = Feat<TRACE, FISSION>;
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int s_has_work;
   typedef typename std::conditional<LDS_TALLY, Real, double>::type Acc;
@@ -559,27 +714,22 @@ __global__ __launch_bounds__(kBlock) void k_woodcock(const DevProblem *probs, in
     if (pi != loaded) {
       const uint32_t *src = (const uint32_t *)P.tab;
       uint32_t *dst = (uint32_t *)smem;
-      for (int i = threadIdx.x; i < P.lay.bytes / 4; i += blockDim.x) dst[i] = src[i];
+      for (int i = threadIdx.x; i < P.lay.hot / 4; i += BS) dst[i] = src[i];
       loaded = pi;
     }
     const int TB = P.tally_bins;
     if (LDS_TALLY)
-      for (int i = threadIdx.x; i < TB; i += blockDim.x) ltally[i] = Real(0);
+      for (int i = threadIdx.x; i < TB; i += BS) ltally[i] = Real(0);
     __syncthreads();
 
-    Tabs<Real> T;
-    T.bind(smem, P.lay);
-    const int nc = P.nc;
+    Ctx<Real> C;
+    C.load(P, cs, smem);
     Acc *direct = LDS_TALLY ? (Acc *)ltally : (Acc *)P.tally_glob;
-    Acc *diff = direct + P.G * nc;
-    const Real maj = (Real)P.maj, inv_maj = (Real)P.inv_maj;
-    const Real XL = (Real)P.XL, XR = (Real)P.XR, bl = (Real)P.boundl, br = (Real)P.boundr;
-    const Real inv_k = (Real)cs.inv_k;
-    const int64_t N = P.N;
-    int top = 1;
-    while (top * 2 < nc) top *= 2;
-    LocalStats st;
-    for (int i = 0; i < kNumStats; i++) st.v[i] = 0;
+    Acc *diff = direct + C.G * C.nc;
+    unsigned long long *queue = (unsigned long long *)&P.cs->queue_head;
+    WaveStats ws;
+    SiteCursor sites;
+    uint32_t ncross = 0;
 
     Particle<Real> q;
     LaneRng rng;
@@ -587,48 +737,56 @@ __global__ __launch_bounds__(kBlock) void k_woodcock(const DevProblem *probs, in
     int64_t r_cur = 0, r_end = 0;  // wave-uniform claimed range
     for (;;) {
       uint64_t need = __ballot(!active);
-      while (need && !queue_done) {
-        if (r_cur >= r_end) {
-          unsigned long long b = 0;
-          if (lane_id() == 0) b = atomicAdd((unsigned long long *)&P.cs->queue_head,
-                                            (unsigned long long)claim);
-          b = __shfl(b, 0);
-          if ((int64_t)b >= N) {
-            queue_done = true;
-            break;
+      if (need && !queue_done && (__popcll(need) >= refill_min || need == ~0ull)) {
+        while (need && !queue_done) {
+          if (r_cur >= r_end) {
+            unsigned long long b = 0;
+            if (lane_id() == 0) b = atomicAdd(queue, (unsigned long long)claim);
+            b = __shfl(b, 0);
+            if ((int64_t)b >= C.N) {
+              queue_done = true;
+              break;
+            }
+            r_cur = (int64_t)b;
+            r_end = r_cur + claim < C.N ? r_cur + claim : C.N;
           }
-          r_cur = (int64_t)b;
-          r_end = r_cur + claim < N ? r_cur + claim : N;
-        }
-        const int cnt = __popcll(need);
-        const int64_t avail = r_end - r_cur;
-        const int take = avail < cnt ? (int)avail : cnt;
-        const int rk = rank_below(need);
-        if (!active && rk < take) {
-          const int64_t p = r_cur + rk;
-          init_particle(P, cs, p, rng, q);
-          birth(P, T, cs.from_bank, p, rng, q);
-          q.gxs = q.g;
-          q.rxs = T.reg(q.cell);
-          trace_event(P, q, NRAPS_EV_BIRTH, q.cell, q.g, 0, q.x, q.mu);
-          active = true;
-        }
-        r_cur += take;
-        need = __ballot(!active);
-      }
-      if (!__any(active)) break;
-      if (active) {
-        if (!woodcock_step<Real, Acc>(P, T, cs, inv_k, maj, inv_maj, XL, XR, bl, br, top, direct,
-                                      diff, rng, q, st)) {
-          finish(P, cs, q, rng);
-          active = false;
+          const int cnt = __popcll(need);
+          const int64_t avail = r_end - r_cur;
+          const int take = avail < cnt ? (int)avail : cnt;
+          const int rk = rank_below(need);
+          if (!active && rk < take) {
+            const int64_t p = r_cur + rk;
+            init_particle<F>(C, p, rng, q);
+            birth(C, p, rng, q);
+            q.gxs = q.g;
+            q.rxs = C.T.reg(q.cell);
+            q.kcur = kint(C, q.g, q.x, q.cell);
+            trace_event<F>(C, q, NRAPS_EV_BIRTH, q.cell, q.g, 0, q.x, q.mu);
+            active = true;
+          }
+          r_cur += take;
+          need = __ballot(!active);
         }
       }
+      if (!__any(active)) {
+        if (queue_done) break;
+        continue;
+      }
+      uint32_t ev = 0;
+      int nsite = 0;
+      if (active) ev = woodcock_step<F, Real, Acc>(C, direct, diff, rng, q, ncross, nsite);
+      if (F::FISSION && C.make_sites) sites.emit(C, nsite, q);  // wave-uniform
+      if (active && (ev & kEvDead)) {
+        finish<F>(C, q, rng);
+        active = false;
+      }
+      ws.add(ev);
     }
-    flush_stats(P, st);
+    if (F::FISSION && C.make_sites) sites.close(C);
+    ws.flush(P, ncross);
     __syncthreads();
     if (LDS_TALLY) {
-      for (int i = threadIdx.x; i < TB; i += blockDim.x) {
+      for (int i = threadIdx.x; i < TB; i += BS) {
         Real v = ltally[i];
         if (v != Real(0)) atomicAdd(&P.tally_glob[i], (double)v);
       }
@@ -637,10 +795,19 @@ __global__ __launch_bounds__(kBlock) void k_woodcock(const DevProblem *probs, in
   }
 }
 
-template __global__ void k_woodcock<float, true>(const DevProblem *, int, int, int);
-template __global__ void k_woodcock<float, false>(const DevProblem *, int, int, int);
-template __global__ void k_woodcock<double, true>(const DevProblem *, int, int, int);
-template __global__ void k_woodcock<double, false>(const DevProblem *, int, int, int);
+#define NRAPS_WC_INST(R, L, BS)                                                                \
+  template __global__ void k_woodcock<R, L, false, false, BS>(const DevProblem *, int, int, int, \
+                                                              int);                             \
+  template __global__ void k_woodcock<R, L, false, true, BS>(const DevProblem *, int, int, int,  \
+                                                             int);                              \
+  template __global__ void k_woodcock<R, L, true, true, BS>(const DevProblem *, int, int, int, int);
+NRAPS_WC_INST(float, true, 256)
+NRAPS_WC_INST(float, false, 256)
+NRAPS_WC_INST(float, true, 1024)
+NRAPS_WC_INST(double, true, 256)
+NRAPS_WC_INST(double, false, 256)
+NRAPS_WC_INST(double, true, 1024)
+#undef NRAPS_WC_INST
 
 
 // ---------------------------------------------------------------------------
@@ -726,7 +893,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_tilesums(const DevProblem *prob
     CycleState *cs = P.cs;
     cs->n_sites = run;
     if (run == 0) cs->error |= 2;
-    if ((int64_t)cs->site_count > P.site_cap || (int64_t)cs->site_count != run) cs->error |= 1;
+    // site_count counts reserved slots (chunk tails included): it bounds
+    // the buffer scan of k_place; overflow if it exceeds the capacity
+    if ((int64_t)cs->site_count > P.site_cap || (int64_t)cs->site_count < run) cs->error |= 1;
   }
   __syncthreads();
   int64_t run = part[threadIdx.x];
@@ -779,10 +948,11 @@ __global__ __launch_bounds__(kBlock) void k_place(const DevProblem *probs) {
   const DevProblem &P = probs[blockIdx.y];
   const CycleState *cs = P.cs;
   if (!cs->active || !cs->make_sites || (cs->error & 1)) return;
-  const int64_t n = cs->n_sites;
+  const int64_t n = (int64_t)cs->site_count;  // reserved slots, <= site_cap
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
     uint64_t key = P.site_key[i];
+    if (key == ~0ull) continue;  // unused tail of a wave's slot chunk
     int64_t dst = P.offsets[key >> 32] + (int64_t)(key & 0xffffffffu);
     ((Real *)P.ord_x)[dst] = ((const Real *)P.site_x)[i];
     P.ord_cell[dst] = P.site_cell[i];
@@ -796,7 +966,7 @@ __global__ __launch_bounds__(kBlock) void k_resample(const DevProblem *probs) {
   if (!cs->active || !cs->make_sites || (cs->error & 3)) return;
   const int64_t M = cs->n_sites;
   nraps_pcg32 r;
-  nraps_pcg32_seed(&r, P.seed, nraps_stream_id(P.problem_id, (uint64_t)cs->cycle, NRAPS_SERVICE_PARTICLE));
+  nraps_history_rng(&r, P.seed, nraps_stream_id(P.problem_id, (uint64_t)cs->cycle, NRAPS_SERVICE_PARTICLE));
   const double u0 = nraps_u01_f64(nraps_pcg32_next(&r));
   const double ratio = (double)M / (double)P.N;
   for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < P.N;
@@ -936,38 +1106,43 @@ void launch_surface(const DevProblem *d, int nprob, int64_t max_n, hipStream_t s
   hipLaunchKernelGGL(k_surface, grid, dim3(kBlock), 0, s, d, nprob);
 }
 
-template <typename Real, bool L>
-static void launch_wc(const DevProblem *d, int nprob, int grid, size_t lds, int tab, int claim,
-                      hipStream_t s) {
-  hipLaunchKernelGGL((k_woodcock<Real, L>), dim3((unsigned)grid), dim3(kBlock), lds, s, d, nprob, tab,
-                     claim);
-}
-
-void launch_woodcock(bool fp32, bool lds_tally, const DevProblem *d, int nprob, int grid,
-                     size_t lds_bytes, int tab_lds_bytes, int claim, hipStream_t s) {
-  if (fp32 && lds_tally) launch_wc<float, true>(d, nprob, grid, lds_bytes, tab_lds_bytes, claim, s);
-  if (fp32 && !lds_tally) launch_wc<float, false>(d, nprob, grid, lds_bytes, tab_lds_bytes, claim, s);
-  if (!fp32 && lds_tally) launch_wc<double, true>(d, nprob, grid, lds_bytes, tab_lds_bytes, claim, s);
-  if (!fp32 && !lds_tally) launch_wc<double, false>(d, nprob, grid, lds_bytes, tab_lds_bytes, claim, s);
-}
-
-template <typename Real, bool L>
+// Variant of the event kernel: precision, tally placement, and the
+// compile-time features (traces imply the fission code path).
+template <typename Real, bool L, bool T, bool Fi, int BS>
 static const void *wc_fn() {
-  return reinterpret_cast<const void *>(&k_woodcock<Real, L>);
+  return reinterpret_cast<const void *>(&k_woodcock<Real, L, T, Fi, BS>);
 }
-static const void *wc_select(bool fp32, bool lds_tally) {
-  if (fp32) return lds_tally ? wc_fn<float, true>() : wc_fn<float, false>();
-  return lds_tally ? wc_fn<double, true>() : wc_fn<double, false>();
+template <typename Real, bool L, int BS>
+static const void *wc_pick(bool trace, bool fission) {
+  if (trace) return wc_fn<Real, L, true, true, BS>();
+  return fission ? wc_fn<Real, L, false, true, BS>() : wc_fn<Real, L, false, false, BS>();
+}
+template <typename Real>
+static const void *wc_pick_mode(const WcVariant &v) {
+  if (v.big_block) return wc_pick<Real, true, 1024>(v.trace, v.fission);
+  return v.lds_tally ? wc_pick<Real, true, 256>(v.trace, v.fission)
+                     : wc_pick<Real, false, 256>(v.trace, v.fission);
+}
+static const void *wc_select(const WcVariant &v) {
+  return v.fp32 ? wc_pick_mode<float>(v) : wc_pick_mode<double>(v);
 }
 
-hipError_t woodcock_set_lds_limit(bool fp32, bool lds_tally, size_t lds_bytes) {
-  return hipFuncSetAttribute(wc_select(fp32, lds_tally), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds_bytes);
+void launch_woodcock(const WcVariant &v, const DevProblem *d, int nprob, int grid, size_t lds_bytes,
+                     int tab_lds_bytes, int claim, int refill_min, hipStream_t s) {
+  void *args[] = {(void *)&d, (void *)&nprob, (void *)&tab_lds_bytes, (void *)&claim,
+                  (void *)&refill_min};
+  hipError_t e = hipLaunchKernel(wc_select(v), dim3((unsigned)grid), dim3(v.block_size()), args,
+                                 lds_bytes, s);
+  (void)e;  // reported by the caller's hipGetLastError
 }
 
-int woodcock_blocks_per_cu(bool fp32, bool lds_tally, size_t lds_bytes) {
+hipError_t woodcock_set_lds_limit(const WcVariant &v, size_t lds_bytes) {
+  return hipFuncSetAttribute(wc_select(v), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+}
+
+int woodcock_blocks_per_cu(const WcVariant &v, size_t lds_bytes) {
   int nb = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, wc_select(fp32, lds_tally), kBlock, lds_bytes) !=
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, wc_select(v), v.block_size(), lds_bytes) !=
       hipSuccess)
     return 1;
   return nb > 0 ? nb : 1;

@@ -53,12 +53,30 @@ NRAPS_HD void nraps_pcg32_seed(nraps_pcg32 *r, uint64_t seed, uint64_t stream) {
   (void)nraps_pcg32_next(r);
 }
 
-/* Stream id of particle p in generation `cycle` of problem `problem`.
-   Every history owns one stream, so its draws do not depend on scheduling,
+/* Id of particle p in generation `cycle` of problem `problem`.  Every
+   history owns one generator, so its draws do not depend on scheduling,
    thread count or batch composition.  Requires problem < 2^19,
    cycle < 2^16, p < 2^28. */
 NRAPS_HD uint64_t nraps_stream_id(uint64_t problem, uint64_t cycle, uint64_t p) {
   return (problem << 44) | (cycle << 28) | p;
+}
+
+/* SplitMix64 output function (Steele, Lea & Flood, "Fast splittable
+   pseudorandom number generators", OOPSLA 2014, with the multipliers of
+   D. Stafford's Mix13): a bijection of 64-bit words. */
+NRAPS_HD uint64_t nraps_mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+#define NRAPS_GOLDEN64 0x9E3779B97F4A7C15ULL
+
+/* Generator of one history: PCG32 initialised as Prn32::new, with both the
+   initial state and the stream scrambled from the history id.  Ids that
+   differ only in high bits (the same particle in another generation) would
+   otherwise give LCG states that agree in their low bits at every step. */
+NRAPS_HD void nraps_history_rng(nraps_pcg32 *r, uint64_t seed, uint64_t id) {
+  nraps_pcg32_seed(r, seed ^ nraps_mix64(id), nraps_mix64(id + NRAPS_GOLDEN64));
 }
 /* Per-(problem, cycle) service stream (fission-bank resampling offset). */
 #define NRAPS_SERVICE_PARTICLE ((1u << 28) - 1u)

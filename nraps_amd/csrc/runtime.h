@@ -8,6 +8,7 @@
 
 #include "device.h"
 #include "host.h"
+#include "launch.h"
 
 namespace nraps {
 
@@ -93,11 +94,12 @@ class Launcher {
  private:
   void drop_graph();
   int device_;
-  int n_ = 0, n_cu_ = 0, lds_limit_ = 0, blocks_per_cu_ = 1, grid_ = 1, claim_ = 256;
+  int n_ = 0, n_cu_ = 0, lds_limit_ = 0, blocks_per_cu_ = 1, grid_ = 1, claim_ = 256, refill_min_ = 1;
   int tab_lds_ = 0, max_bins_ = 0, levels_ = 1;
   int64_t max_n_ = 0, max_tiles_ = 0;
   size_t lds_bytes_ = 0;
   bool fp32_ = false, woodcock_ = false, fission_any_ = false, lds_tally_ = false;
+  WcVariant variant_{};
   DevProblem *d_probs_ = nullptr;
   DevFinal *d_fins_ = nullptr;
   int64_t *d_tile_sums_ = nullptr, *d_tile_base_ = nullptr;

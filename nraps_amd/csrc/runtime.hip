@@ -9,6 +9,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -41,6 +42,14 @@ void *DevArena::raw(size_t bytes) {
 
 static int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+static int env_int(const char *name, int dflt) {
+  const char *v = std::getenv(name);
+  if (!v || !*v) return dflt;
+  char *e = nullptr;
+  long x = std::strtol(v, &e, 10);
+  return (e && *e == '\0') ? (int)x : dflt;
+}
+
 int ktree_levels(int64_t n) {
   int L = 0;
   do {
@@ -50,7 +59,9 @@ int ktree_levels(int64_t n) {
   return L;
 }
 
-int64_t site_capacity(int64_t N) { return 4 * N + 65536; }
+// Unordered site-buffer slots: up to four sites per history plus the
+// unused tails of the waves' 128-slot chunks (a few thousand waves).
+int64_t site_capacity(int64_t N) { return 4 * N + (1 << 20); }
 
 int64_t problem_footprint(const nraps_problem *pb, const nraps_run_opts *o) {
   const int64_t N = o->histories, nc = pb->n_cells, G = pb->n_groups;
@@ -97,10 +108,12 @@ void ProblemDev::setup(const nraps_problem *pb_, const nraps_run_opts &o_, bool 
   d.xs_every = o.xs_refresh == NRAPS_XS_EVERY_COLLISION;
   d.fission = fission;
   d.tally_bins = 2 * P.G * P.nc;
+  d.nb = P.nb;
+  d.inv_h = P.inv_h;
   d.N = N;
   d.seed = o.seed;
   d.problem_id = o.problem_id;
-  d.max_events = o.max_events > 0 ? o.max_events : 10000000;
+  d.max_events = o.max_events > 0 ? std::min<int64_t>(o.max_events, 2147483647) : 10000000;
   d.boundl = P.boundl;
   d.boundr = P.boundr;
   d.XL = P.XL;
@@ -138,6 +151,7 @@ void ProblemDev::setup(const nraps_problem *pb_, const nraps_run_opts &o_, bool 
     d.trace_len = arena.alloc<int32_t>(trace_n);
     d.draws = arena.alloc<int64_t>(trace_n);
     d.trace_kacc = arena.alloc<double>(trace_n);
+    HIP_TRY(hipMemset(d.trace, 0, sizeof(nraps_trace_event) * trace_n * o.trace_events));
     HIP_TRY(hipMemset(d.trace_len, 0, sizeof(int32_t) * trace_n));
     d.trace_n = trace_n;
     d.trace_ev = o.trace_events;
@@ -328,11 +342,13 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
   levels_ = 1;
   max_tiles_ = 0;
   int tab_max = 0, tally_max = 0;
+  bool trace_any = false;
   int64_t tiles_total = 0, total = 0;
   std::vector<int64_t> tile_base(std::max(n_, 1));
   for (int i = 0; i < n_; i++) {
     const ProblemDev &p = *ps[i];
     fission_any_ |= p.fission;
+    trace_any |= p.trace_n > 0;
     max_n_ = std::max<int64_t>(max_n_, p.o.histories);
     max_bins_ = std::max(max_bins_, p.dev.tally_bins);
     levels_ = std::max(levels_, ktree_levels(p.o.histories));
@@ -340,24 +356,43 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     const int64_t t = ceil_div(p.o.histories, kScanTile);
     tiles_total += t;
     max_tiles_ = std::max(max_tiles_, t);
-    tab_max = std::max(tab_max, p.lay.bytes);
+    tab_max = std::max(tab_max, p.lay.hot);
     tally_max = std::max(tally_max, p.dev.tally_bins * (fp32 ? 4 : 8));
     total += p.o.histories;
   }
   if (woodcock_ && n_ > 0) {
-    // LDS plan: tables always; the tally too when both leave room for at
-    // least two resident blocks per CU.
+    // LDS plan.  The hot tables always go to LDS.  The tally goes there too:
+    // with 256-thread blocks when tables + tally leave room for four blocks
+    // per CU, else with one 1024-thread block per CU owning the whole LDS;
+    // only when neither fits is it scored into the global FP64 tally.
     if (tab_max > lds_limit_)
       throw std::invalid_argument("problem tables (" + std::to_string(tab_max) +
                                   " B) exceed the LDS of one block");
-    const int lds_budget = std::min(lds_limit_, 64 * 1024);
+    // tuning knobs (environment): LDS budgets of the two block shapes, and
+    // the number of idle lanes that triggers a refill
+    const int small_budget = std::min(lds_limit_ / 4, env_int("NRAPS_LDS_SMALL", lds_limit_ / 4));
+    const int large_budget = std::min(lds_limit_ - 256, env_int("NRAPS_LDS_BUDGET", lds_limit_ - 256));
+    refill_min_ = std::max(1, std::min(64, env_int("NRAPS_REFILL_MIN", 16)));
     tab_lds_ = tab_max;
-    lds_tally_ = tab_max + tally_max <= lds_budget;
+    const int need = tab_max + tally_max;
+    variant_ = WcVariant{};
+    if (need <= small_budget) {
+      lds_tally_ = true;
+    } else if (need <= large_budget) {
+      lds_tally_ = true;
+      variant_.big_block = true;
+    } else {
+      lds_tally_ = false;
+    }
     lds_bytes_ = (size_t)tab_max + (lds_tally_ ? (size_t)tally_max : 0);
-    (void)woodcock_set_lds_limit(fp32_, lds_tally_, lds_bytes_);
-    blocks_per_cu_ = woodcock_blocks_per_cu(fp32_, lds_tally_, lds_bytes_);
+    variant_.fp32 = fp32_;
+    variant_.lds_tally = lds_tally_;
+    variant_.fission = fission_any_;
+    variant_.trace = trace_any;
+    (void)woodcock_set_lds_limit(variant_, lds_bytes_);
+    blocks_per_cu_ = woodcock_blocks_per_cu(variant_, lds_bytes_);
     grid_ = n_cu_ * blocks_per_cu_;
-    const int64_t waves = (int64_t)grid_ * (kBlock / 64);
+    const int64_t waves = (int64_t)grid_ * (variant_.block_size() / 64);
     int64_t c = total / (waves * 8);
     c = std::max<int64_t>(64, std::min<int64_t>(1024, c));
     claim_ = (int)((c + 63) / 64 * 64);
@@ -405,7 +440,7 @@ void Launcher::enqueue_cycle(hipStream_t s, hipEvent_t tr0, hipEvent_t tr1) {
   launch_cycle_begin(d_probs_, n_, max_bins_, s);
   if (tr0) HIP_TRY(hipEventRecord(tr0, s));
   if (woodcock_)
-    launch_woodcock(fp32_, lds_tally_, d_probs_, n_, grid_, lds_bytes_, tab_lds_, claim_, s);
+    launch_woodcock(variant_, d_probs_, n_, grid_, lds_bytes_, tab_lds_, claim_, refill_min_, s);
   else
     launch_surface(d_probs_, n_, max_n_, s);
   if (tr1) HIP_TRY(hipEventRecord(tr1, s));

@@ -1,0 +1,69 @@
+"""Profiling driver: run one workload for a few generations, then a
+device-to-device copy of known size (a blit kernel) whose FETCH_SIZE /
+WRITE_SIZE counters calibrate the HBM-byte counters of the run.
+
+    rocprofv3 --kernel-trace --stats -d OUT -o run -f csv -- \\
+        python3 -m nraps_amd.profile_run --config 1 --generations 4
+    rocprofv3 --pmc FETCH_SIZE --kernel-trace -d OUT -o fetch -f csv -- \\
+        python3 -m nraps_amd.profile_run --config 1 --generations 2 --copy-mib 1024
+"""
+import argparse
+import ctypes as C
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import nraps_amd as nr  # noqa: E402
+from nraps_amd import _abi  # noqa: E402
+
+
+def copy_probe(mib):
+    """hipMemcpy device-to-device of `mib` MiB (runs as a blit kernel)."""
+    hip = C.CDLL("libamdhip64.so")
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--index", type=int, default=0)
+    ap.add_argument("--generations", type=int, default=4)
+    ap.add_argument("--histories", type=int, default=0)
+    ap.add_argument("--precision", choices=["fp32", "fp64"], default=None)
+    ap.add_argument("--method", choices=["event", "plain"], default=None)
+    ap.add_argument("--copy-mib", type=int, default=0)
+    a = ap.parse_args()
+    p, o = nr.generate(a.config, a.index)
+    o.generations = max(a.generations, 2)
+    o.skip = 1
+    o.use_graph = 0
+    if a.histories:
+        o.histories = a.histories
+    if a.precision:
+        o.precision = _abi.FP32 if a.precision == "fp32" else _abi.FP64
+    if a.method:
+        o.method = _abi.EVENT_WOODCOCK if a.method == "event" else _abi.HISTORY_SURFACE
+        if o.method == _abi.HISTORY_SURFACE:
+            o.precision = _abi.FP64
+    r = nr.solve(p, o)
+    print(f"config {a.config}: {o.histories} x {o.generations} histories, "
+          f"transport {r.ms_transport / max(r.transport_launches, 1):.3f} ms/launch, "
+          f"k {r.k_mean:.6f}", flush=True)
+    if a.copy_mib:
+        copy_probe(a.copy_mib)
+        print(f"copy probe: {a.copy_mib} MiB device-to-device", flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -81,9 +81,29 @@ uint64_t oracle_stream_id(uint64_t problem, uint64_t cycle, uint64_t p) {
   return problem * (1ULL << 44) + cycle * (1ULL << 28) + p;
 }
 
+/* SplitMix64 finalizer (Steele, Lea & Flood 2014; Stafford's Mix13 constants). */
+uint64_t oracle_mix64(uint64_t z) {
+  z ^= z >> 30;
+  z *= 0xBF58476D1CE4E5B9ULL;
+  z ^= z >> 27;
+  z *= 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+/* Generator of history `id`: state and stream both scrambled from the id. */
+static void orng_history(orng *r, uint64_t seed, uint64_t id) {
+  orng_init(r, seed ^ oracle_mix64(id), oracle_mix64(id + 0x9E3779B97F4A7C15ULL));
+}
+
 void oracle_pcg32_stream(uint64_t seed, uint64_t stream, int32_t n, uint32_t *out) {
   orng r;
   orng_init(&r, seed, stream);
+  for (int32_t i = 0; i < n; i++) out[i] = orng_u32(&r);
+}
+
+void oracle_history_stream(uint64_t seed, uint64_t id, int32_t n, uint32_t *out) {
+  orng r;
+  orng_history(&r, seed, id);
   for (int32_t i = 0; i < n; i++) out[i] = orng_u32(&r);
 }
 
@@ -636,7 +656,7 @@ static void orun_block(ojob *J, int64_t b) {
   nraps_result *res = J->res;
   for (int64_t i = lo; i < hi; i++) {
     orng r;
-    orng_init(&r, cy->o->seed, oracle_stream_id(cy->o->problem_id, cy->cycle, i));
+    orng_history(&r, cy->o->seed, oracle_stream_id(cy->o->problem_id, cy->cycle, i));
     int traced = J->trace_now && i < cy->o->trace_histories;
     w.tr = traced && res->trace ? &res->trace[i * cy->o->trace_events] : NULL;
     w.tr_cap = traced ? cy->o->trace_events : 0;
@@ -801,7 +821,7 @@ static int oracle_solve_impl(const nraps_problem *pb, const nraps_run_opts *o, i
           at += block_sites[b].n;
         }
         orng rs;
-        orng_init(&rs, o->seed, oracle_stream_id(o->problem_id, cyc, NRAPS_SERVICE_PARTICLE));
+        orng_history(&rs, o->seed, oracle_stream_id(o->problem_id, cyc, NRAPS_SERVICE_PARTICLE));
         double u0 = orng_u(&rs);
         double ratio = (double)nsites / (double)N;
         for (int64_t j = 0; j < N; j++) {

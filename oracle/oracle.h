@@ -50,6 +50,10 @@ int32_t oracle_cdf_index(const double *cdf, int32_t n, double u); /* partition_p
 /* PCG32, written independently of the product (src/rand.rs:36-107). */
 void oracle_pcg32_stream(uint64_t seed, uint64_t stream, int32_t n, uint32_t *out);
 uint64_t oracle_stream_id(uint64_t problem, uint64_t cycle, uint64_t p);
+/* SplitMix64 finalizer and the draws of the generator of history `id`
+   (initial state seed ^ mix(id), stream mix(id + golden)). */
+uint64_t oracle_mix64(uint64_t z);
+void oracle_history_stream(uint64_t seed, uint64_t id, int32_t n, uint32_t *out);
 
 /* Canonical fixed-order sum used for the k estimator: 256-wide pairwise
    trees, level by level, zero padded (also implemented on the device). */

@@ -25,6 +25,9 @@ _L.oracle_cdf_index.restype = C.c_int32
 _L.oracle_pcg32_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, C.POINTER(C.c_uint32)]
 _L.oracle_stream_id.argtypes = [C.c_uint64] * 3
 _L.oracle_stream_id.restype = C.c_uint64
+_L.oracle_mix64.argtypes = [C.c_uint64]
+_L.oracle_mix64.restype = C.c_uint64
+_L.oracle_history_stream.argtypes = [C.c_uint64, C.c_uint64, C.c_int32, C.POINTER(C.c_uint32)]
 _L.oracle_canonical_sum.argtypes = [_dp, C.c_int64]
 _L.oracle_canonical_sum.restype = C.c_double
 _L.oracle_log.argtypes = [C.c_double]
@@ -92,6 +95,16 @@ def pcg32(seed, stream, n):
 
 def stream_id(problem, cycle, p):
     return _L.oracle_stream_id(problem, cycle, p)
+
+
+def mix64(z):
+    return _L.oracle_mix64(z)
+
+
+def history_stream(seed, hid, n):
+    out = np.zeros(n, dtype=np.uint32)
+    _L.oracle_history_stream(seed, hid, n, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+    return out
 
 
 def canonical_sum(v):

@@ -1,0 +1,219 @@
+"""GPU tests of the batched and serving forms, edge cases and statistics.
+
+k per generation is a canonical fixed-order FP64 sum of per-history
+contributions and each history owns its PCG32 stream, so a problem's k
+sequence is bit-identical alone, in a batch, in the server's mix, with or
+without graph replay.  Fluxes are atomically accumulated (FP32 per-block
+partials in the precision form) and are compared to a stated tolerance.
+"""
+import math
+import time
+
+import numpy as np
+import pytest
+
+import nraps_amd as nr
+from nraps_amd import _abi
+
+pytestmark = pytest.mark.gpu
+
+
+def small_c3(i, histories=20000, generations=8, skip=3):
+    p, o = nr.generate(3, i)
+    o.histories = min(int(o.histories), histories)
+    o.generations, o.skip = generations, skip
+    return p, o
+
+
+def assert_same_k(a, b):
+    assert np.array_equal(a.k.view(np.uint64), b.k.view(np.uint64))
+
+
+def test_batch_equals_individual_solves(gpu):
+    probs = [small_c3(i) for i in range(5)]
+    solo = [nr.solve(p, o) for p, o in probs]
+    batch = nr.solve_batch([p for p, _ in probs], [o for _, o in probs])
+    for s, b in zip(solo, batch):
+        assert_same_k(s, b)
+        np.testing.assert_allclose(b.flux_mean, s.flux_mean, rtol=2e-5, atol=1e-9)
+        assert np.array_equal(s.sites, b.sites)
+
+
+def test_graph_replay_equals_direct_launches(gpu):
+    p, o = small_c3(2)
+    o.use_graph = 1
+    a = nr.solve(p, o)
+    o.use_graph = 0
+    b = nr.solve(p, o)
+    assert_same_k(a, b)
+
+
+def test_server_results_equal_alone_and_in_the_mix(gpu):
+    probs = [small_c3(i, histories=15000, generations=10, skip=4) for i in range(8)]
+    solo = [nr.solve(p, o) for p, o in probs]
+    with nr.Server(precision=_abi.FP32, budget_bytes=8 * 10**9, max_slots=4) as srv:
+        tickets = []
+        for j, (p, o) in enumerate(probs):
+            tickets.append(srv.submit(p, o))
+            if j == 3:
+                time.sleep(0.01)  # later arrivals join a running mix
+        res = [srv.wait(t, timeout_ms=120000) for t in tickets]
+        lat = [srv.latency(t) for t in tickets]
+    for s, r, (st, q, tot, cyc) in zip(solo, res, lat):
+        assert r.status_code == 0 and st == 0
+        assert cyc == 10 and tot >= q >= 0
+        assert_same_k(s, r)
+        np.testing.assert_allclose(r.flux_mean, s.flux_mean, rtol=2e-5, atol=1e-9)
+
+
+def test_server_budget_admission_and_rejection(gpu):
+    p, o = small_c3(1, histories=20000, generations=6, skip=3)
+    with nr.Server(precision=_abi.FP32, budget_bytes=3 * 10**8, max_slots=8) as srv:
+        big, _ = nr.generate(3, 1)
+        o_big = nr.run_opts(histories=4 * 10**6, generations=6, skip=3,
+                            source=_abi.SOURCE_FISSION_BANK)
+        with pytest.raises(nr.NrapsError) as e:
+            srv.submit(big, o_big)  # its banks alone exceed the budget
+        assert e.value.status == _abi.NRAPS_E_OVER_BUDGET
+        ts = [srv.submit(p, o) for _ in range(3)]
+        for t in ts:
+            assert srv.wait(t, timeout_ms=120000).status_code == 0
+
+
+def test_server_sigma_stop(gpu):
+    p, o = small_c3(0, histories=20000, generations=400, skip=3)
+    with nr.Server(precision=_abi.FP32, budget_bytes=8 * 10**9, max_slots=4,
+                   sigma_k_target=5e-3, min_active=5) as srv:
+        t1 = srv.submit(p, o)
+        r1 = srv.wait(t1, timeout_ms=120000)
+        st, q, tot, cycles = srv.latency(t1)
+        assert r1.status_code == 0 and 3 + 5 <= cycles < 400 and r1.k_sigma <= 5e-3
+        assert r1.histories_run == cycles * o.histories
+
+
+def test_server_cancel(gpu):
+    p, o = small_c3(0, histories=20000, generations=60000, skip=3)
+    with nr.Server(precision=_abi.FP32, budget_bytes=8 * 10**9, max_slots=4) as srv:
+        t = srv.submit(p, o)
+        time.sleep(0.2)
+        assert srv.poll(t).status_code == _abi.NRAPS_PENDING
+        srv.cancel(t)
+        r = srv.wait(t, timeout_ms=120000)
+        assert r.status_code == _abi.NRAPS_E_CANCELLED
+        # the server keeps serving after a cancellation
+        o.generations = 6
+        assert srv.wait(srv.submit(p, o), timeout_ms=120000).status_code == 0
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 257])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+def test_ragged_history_counts(gpu, oracle, n, method):
+    p, _ = nr.generate(0)
+    o = nr.run_opts(histories=n, generations=3, skip=1, method=method, precision=_abi.FP64,
+                    trace_histories=n, trace_events=64, trace_cycle=2)
+    g = nr.solve(p, o, trace=True)
+    c = oracle.solve(p, o, threads=2, trace=True)
+    assert_same_k(g, c)
+    assert np.array_equal(g.trace_len[:n], c.trace_len[:n])
+    for i in range(n):
+        assert g.trace_of(i, 64).tobytes() == c.trace_of(i, 64).tobytes()
+    # slots past a history's recorded events stay zero
+    assert g.trace[: n * 64].tobytes() == c.trace[: n * 64].tobytes()
+
+
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+def test_event_cap_matches_oracle(gpu, oracle, method):
+    p, _ = nr.generate(0)
+    o = nr.run_opts(histories=3000, generations=2, skip=1, method=method, max_events=3)
+    g = nr.solve(p, o, check=False)
+    c = oracle.solve(p, o, threads=2)
+    assert g.status_code == c.status_code == _abi.NRAPS_E_EVENT_CAP
+    assert g.capped == c.capped > 0
+    assert g.absorptions + g.leaks + g.capped == g.histories_run
+    assert_same_k(g, c)
+
+
+def test_testcase_c_deck_plain_port_matches_oracle(gpu, oracle):
+    from conftest import REFDATA
+    p, _ = nr.load_deck(REFDATA + "/TestCaseC.txt")
+    o = nr.run_opts(histories=4000, generations=3, skip=1)
+    g = nr.solve(p, o)
+    c = oracle.solve(p, o, threads=4)
+    assert_same_k(g, c)
+    np.testing.assert_allclose(g.flux, c.flux, rtol=1e-10)
+
+
+def _seed_stats(run, seeds):
+    """Mean and standard error over independent seeds of k and of the four
+    group-integrated fluxes (bins of one run are correlated, so the error of
+    a group total is estimated from independent runs)."""
+    ks, tots = [], []
+    for s in seeds:
+        r = run(s)
+        ks.append(r.k_mean)
+        tots.append(r.flux_mean.reshape(4, -1).sum(1))
+    ks, tots = np.array(ks), np.array(tots)
+    n = len(seeds)
+    return ks.mean(), ks.std(ddof=1) / math.sqrt(n), tots.mean(0), tots.std(0, ddof=1) / math.sqrt(n)
+
+
+def test_statistical_agreement_c0(gpu, oracle):
+    """FP32 event, FP64 event and the plain FP64 port against the FP64
+    oracle on the c0 problem, eight independent seeds each: k and the group
+    fluxes within 4 combined standard errors and 0.5%; the FP32 and FP64
+    event paths on the same streams agree to 1e-4."""
+    p, _ = nr.generate(0)
+    seeds = range(101, 109)
+    ref = _seed_stats(lambda s: oracle.solve(
+        p, nr.run_opts(histories=100000, generations=12, skip=2, seed=s), threads=7), seeds)
+    modes = {"fp32 event": dict(method=_abi.EVENT_WOODCOCK, precision=_abi.FP32),
+             "fp64 event": dict(method=_abi.EVENT_WOODCOCK, precision=_abi.FP64),
+             "fp64 plain": dict(method=_abi.HISTORY_SURFACE, precision=_abi.FP64)}
+    got = {}
+    for name, kw in modes.items():
+        got[name] = st = _seed_stats(lambda s: nr.solve(
+            p, nr.run_opts(histories=10**6, generations=12, skip=2, seed=s, **kw)), seeds)
+        assert abs(st[0] - ref[0]) < 4 * math.hypot(st[1], ref[1]), name
+        assert np.all(np.abs(st[2] - ref[2]) < 4 * np.hypot(st[3], ref[3])), name
+        assert np.all(np.abs(st[2] / ref[2] - 1) < 5e-3), name
+    a, b = got["fp32 event"], got["fp64 event"]
+    assert abs(a[0] / b[0] - 1) < 1e-4 and np.all(np.abs(a[2] / b[2] - 1) < 1e-4)
+
+
+def test_precision_form_c2_fp32_vs_fp64(gpu):
+    """Precision form on the 201-region 8-group fission-bank slab: FP32
+    tracking/tallies with FP64 k and bank against the FP64 event path, four
+    independent seeds each: k and the eight group fluxes within 4 combined
+    standard errors, group fluxes within 0.5%."""
+    p, _ = nr.generate(2)
+    kw = dict(histories=2 * 10**6, generations=14, skip=4, method=_abi.EVENT_WOODCOCK,
+              source=_abi.SOURCE_FISSION_BANK)
+    stats = {}
+    for prec in (_abi.FP64, _abi.FP32):
+        ks, tots = [], []
+        for s in range(11, 15):
+            r = nr.solve(p, nr.run_opts(precision=prec, seed=s, **kw))
+            ks.append(r.k_mean)
+            tots.append(r.flux_mean.reshape(8, -1).sum(1))
+        ks, tots = np.array(ks), np.array(tots)
+        stats[prec] = (ks.mean(), ks.std(ddof=1) / 2, tots.mean(0), tots.std(0, ddof=1) / 2)
+    a, b = stats[_abi.FP64], stats[_abi.FP32]
+    assert abs(a[0] - b[0]) < 4 * math.hypot(a[1], b[1])
+    assert np.all(np.abs(a[2] - b[2]) < 4 * np.hypot(a[3], b[3]))
+    assert np.all(np.abs(b[2] / a[2] - 1) < 5e-3)
+
+
+def test_full_size_c1_invariants(gpu):
+    """At the bench size (10^7 histories per generation): every history ends
+    exactly once, k from the per-history FP64 sums agrees with k recomputed
+    from the FP32 tallies, and the FP64 event path gives the same k within
+    its statistics."""
+    p, o = nr.generate(1)
+    o.generations, o.skip = 4, 1
+    r = nr.solve(p, o)
+    assert r.absorptions + r.leaks + r.capped == r.histories_run == 4 * 10**7
+    assert r.capped == 0
+    np.testing.assert_allclose(r.k, r.k_ref, rtol=2e-5)
+    o.precision = _abi.FP64
+    r64 = nr.solve(p, o)
+    assert abs(r64.k_mean - r.k_mean) < 5 * math.hypot(r.k_sigma, r64.k_sigma)

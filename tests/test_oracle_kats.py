@@ -125,6 +125,29 @@ def test_pcg32_is_rand_rs_initialisation(oracle):
     assert list(oracle.pcg32(42, sid, 8)) == stream(42, sid, 8)
 
 
+def test_splitmix64_published_vectors(oracle):
+    # SplitMix64 (state += 0x9E3779B97F4A7C15; output = mix(state)), seed 0
+    # and seed 1234567, as published with the reference implementation
+    g = 0x9E3779B97F4A7C15
+    M = (1 << 64) - 1
+    for seed, want in ((0, [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]),
+                       (1234567, [0x599ED017FB08FC85, 0x2C73F08458540FA5, 0x883EBCE5A3F27C77])):
+        got = [oracle.mix64((seed + (i + 1) * g) & M) for i in range(3)]
+        assert got == want
+
+
+def test_history_generator_is_pcg32_with_scrambled_seed(oracle):
+    g = 0x9E3779B97F4A7C15
+    M = (1 << 64) - 1
+    hid = oracle.stream_id(2, 5, 999)
+    want = oracle.pcg32(42 ^ oracle.mix64(hid), oracle.mix64((hid + g) & M), 8)
+    assert np.array_equal(oracle.history_stream(42, hid, 8), want)
+    # the same particle in consecutive generations gets unrelated draws
+    a = oracle.history_stream(42, oracle.stream_id(0, 3, 7), 4096).astype(np.float64)
+    b = oracle.history_stream(42, oracle.stream_id(0, 4, 7), 4096).astype(np.float64)
+    assert abs(np.corrcoef(a, b)[0, 1]) < 4 / math.sqrt(4096)
+
+
 def test_next_f32_mapping():
     # Prn32::next_f32 = (u as f32) * 2.3283064E-10; that literal is 2^-32 in f32
     assert np.float32(2.3283064e-10) == np.float32(2.0 ** -32)
