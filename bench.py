@@ -25,6 +25,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 B_STATE_FP32, B_STATE_FP64 = 24, 32
+# HBM bytes per event-kernel launch on this workload, from two rocprofv3 counter
+# passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors calibrated on a
+# 1 GiB device copy): 89.3 KiB x 2 read + 96734 KiB written, per launch of
+# 10^7 histories (profiles/k_woodcock_c1_hbm_traffic.md).
+TRAFFIC_BYTES_PER_LAUNCH = (89.3 * 2 + 96734.0) * 1024
 
 
 def main():
@@ -34,8 +39,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 port leg")
-    ap.add_argument("--cpu-histories", type=int, default=1_000_000)
-    ap.add_argument("--cpu-generations", type=int, default=12)
+    ap.add_argument("--cpu-histories", type=int, default=10_000_000)
+    ap.add_argument("--cpu-generations", type=int, default=24)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -92,7 +97,7 @@ def main():
     avg_ms = tr_ms / max(tr_launch, 1)
     achieved = b_hist * N / (avg_ms * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                       "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
                        "kernel": "k_woodcock<float,lds>", "avg_launch_ms": avg_ms,
                        "bytes_per_history": b_hist, "mean_flights": fbar}
 

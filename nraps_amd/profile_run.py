@@ -45,8 +45,9 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0)
     a = ap.parse_args()
     p, o = nr.generate(a.config, a.index)
-    o.generations = max(a.generations, 2)
-    o.skip = 1
+    fission = o.source == _abi.SOURCE_FISSION_BANK
+    o.generations = max(a.generations, 3 if fission else 2)
+    o.skip = 2 if fission else 1
     o.use_graph = 0
     if a.histories:
         o.histories = a.histories
