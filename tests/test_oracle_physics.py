@@ -198,3 +198,18 @@ def test_event_cap_stops_histories(oracle):
     r = oracle.solve(p, o, threads=2)
     assert r.status_code == _abi.NRAPS_E_EVENT_CAP
     assert r.capped > 0 and r.absorptions + r.leaks + r.capped == r.histories_run
+
+
+def test_generations_are_uncorrelated(oracle):
+    """With the uniform fuel source every generation is an independent
+    sample, so the per-generation k series must show no serial correlation:
+    lags 1..5 of its autocorrelation, n * sum(rho^2) ~ chi^2(5), below the
+    99% point 15.09.  (Streams seeded from raw (cycle, particle) ids share
+    their low LCG state bits between generations; the scrambled seeding
+    exists to remove that.)"""
+    p, _ = nr.generate(0)
+    r, _ = run(oracle, p, histories=20000, generations=401, skip=1, seed=42,
+               method=_abi.EVENT_WOODCOCK)
+    k = np.asarray(r.k[1:]) - np.mean(r.k[1:])
+    rho = [np.dot(k[:-lag], k[lag:]) / np.dot(k, k) for lag in range(1, 6)]
+    assert len(k) * float(np.sum(np.square(rho))) < 15.09
