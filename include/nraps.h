@@ -248,7 +248,9 @@ typedef struct nraps_server_opts {
   int64_t budget_bytes;     /* admission limit on bank + tally bytes */
   int32_t max_slots;        /* problems resident at once */
   int32_t min_active;       /* active cycles before the sigma stop applies */
-  double sigma_k_target;    /* stop a request once its k SEM <= this (0 => off) */
+  double sigma_k_target;    /* stop a request once its relative k standard error,
+                               SEM(k) / |mean k|, is <= this (50 pcm = 5e-4;
+                               0 => off) */
   int32_t use_graph;
   int32_t reserved[7];
 } nraps_server_opts;

@@ -200,7 +200,11 @@ struct Particle {
   int32_t cell, g, gxs, rxs;
   int32_t p, events;
   double kacc;
-  double kcur;  // k integral at (x, cell, g), carried between Woodcock steps
+  // Woodcock tracking scores a straight segment once, when it ends (real
+  // collision, boundary, cap): its start point and the k integral there
+  Real sx;
+  int32_t scell;
+  double ks;
   int32_t nsites, tr;
   bool traced;
 };
@@ -551,91 +555,93 @@ __device__ inline double kint(const Ctx<Real> &C, int g, Real x, int c) {
   return T.kreg[g * (C.R + 1) + r] + T.nsfd[T.mat(c) + C.M * g] * ((double)x - T.reglo[r]);
 }
 
-// One Woodcock step: flight to a tentative site (folding reflections),
-// track-length tally, XS refresh when the path left the region of the last
-// refresh, acceptance test, and the collision if real.  Returns kEv* bits;
-// boundary crossings are added to `ncross`, fission sites to bank at the
-// particle's position are returned in `nsite`.
+// Close the straight segment that started at (q.sx, q.scell) at (xe, ce):
+// its track-length tally and its k contribution, both with weight 1/|mu|.
+// The next segment starts there with k integral `kb`.
+template <typename Real, typename Acc>
+__device__ inline void score_segment(const Ctx<Real> &C, Acc *direct, Acc *diff, Particle<Real> &q,
+                                     Real xe, int ce) {
+  const Real w = Num<Real>::rcp(fabs(q.mu));
+  tally_piece<Real, Acc>(direct, diff, C.T.edges, q.g * C.nc, q.sx, q.scell, xe, ce, w);
+  const double kb = kint(C, q.g, xe, ce);
+  if (q.sx != xe) q.kacc += fabs(kb - q.ks) * (double)w;
+  q.ks = kb;
+  q.sx = xe;
+  q.scell = ce;
+}
+
+// One Woodcock step: flight to a tentative site (folding reflections), XS
+// refresh when the path left the region of the last refresh, acceptance
+// test, and the collision if real.  Virtual collisions change neither the
+// direction nor the group, so the path between real events is one straight
+// segment: it is scored (tally and k) only where it ends, at a boundary, a
+// real collision or the event cap.  Returns kEv* bits; boundary crossings
+// are added to `ncross`, fission sites to bank at the particle's position
+// are returned in `nsite`.
 template <class F, typename Real, typename Acc>
 __device__ inline uint32_t woodcock_step(const Ctx<Real> &C, Acc *direct, Acc *diff, LaneRng &rng,
                                          Particle<Real> &q, uint32_t &ncross, int &nsite) {
   const Tabs<Real> &T = C.T;
   const int nc = C.nc;
   if (++q.events > C.max_events) {
+    score_segment<Real, Acc>(C, direct, diff, q, q.x, q.cell);
     trace_event<F>(C, q, NRAPS_EV_CAP, q.cell, q.g, 0, q.x, q.mu);
     return kEvCap;
   }
-  Real mu = q.mu, x = q.x;
+  Real x = q.x;
   int cell = q.cell;
-  const int g = q.g, gbase = g * nc;
-  Real disp = mu * -Num<Real>::nlog(Num<Real>::flight(rng.next())) * C.inv_maj;
+  Real disp = q.mu * -Num<Real>::nlog(Num<Real>::flight(rng.next())) * C.inv_maj;
   bool left_region = false;
-  Real wgt = Num<Real>::rcp(fabs(mu));
-  double kc = q.kcur;
   for (;;) {
+    const Real mu = q.mu;
     Real end = x + disp;
     if (mu >= Real(0) && end > C.XR) {
-      tally_piece<Real, Acc>(direct, diff, T.edges, gbase, x, cell, C.XR, nc - 1, wgt);
-      double kb = kint(C, g, C.XR, nc - 1);
-      if (x != C.XR) q.kacc += fabs(kb - kc) * (double)wgt;
-      kc = kb;
+      score_segment<Real, Acc>(C, direct, diff, q, C.XR, nc - 1);
       if (T.reg(nc - 1) != q.rxs) left_region = true;
       ncross++;
       if (C.br > Real(0)) {
-        Real m2 = mu * (-C.br);
+        q.mu = mu * (-C.br);
         disp = (disp + (x - C.XR)) * (-C.br);
         x = C.XR;
-        mu = m2;
         cell = nc - 1;
-        wgt = Num<Real>::rcp(fabs(mu));
         continue;
       }
-      trace_event<F>(C, q, NRAPS_EV_LEAK, nc - 1, g, 0, C.XR, mu);
+      trace_event<F>(C, q, NRAPS_EV_LEAK, nc - 1, q.g, 0, C.XR, mu);
       return kEvStep | kEvLeak;
     } else if (mu < Real(0) && C.XL > end) {
-      tally_piece<Real, Acc>(direct, diff, T.edges, gbase, x, cell, C.XL, 0, wgt);
-      double kb = kint(C, g, C.XL, 0);
-      if (x != C.XL) q.kacc += fabs(kb - kc) * (double)wgt;
-      kc = kb;
+      score_segment<Real, Acc>(C, direct, diff, q, C.XL, 0);
       if (T.reg(0) != q.rxs) left_region = true;
       ncross++;
       if (C.bl > Real(0)) {
-        Real m2 = mu * (-C.bl);
+        q.mu = mu * (-C.bl);
         disp = (disp + (x - C.XL)) * (-C.bl);
         x = C.XL;
-        mu = m2;
         cell = 0;
-        wgt = Num<Real>::rcp(fabs(mu));
         continue;
       }
-      trace_event<F>(C, q, NRAPS_EV_LEAK, 0, g, 0, C.XL, mu);
+      trace_event<F>(C, q, NRAPS_EV_LEAK, 0, q.g, 0, C.XL, mu);
       return kEvStep | kEvLeak;
     } else {
       if (end != x) {
-        int ce = locate(C, end, mu >= Real(0));
-        tally_piece<Real, Acc>(direct, diff, T.edges, gbase, x, cell, end, ce, wgt);
-        double kb = kint(C, g, end, ce);
-        q.kacc += fabs(kb - kc) * (double)wgt;
-        kc = kb;
+        cell = locate(C, end, mu >= Real(0));
         x = end;
-        cell = ce;
       }
       break;
     }
   }
   q.x = x;
-  q.mu = mu;
   q.cell = cell;
-  q.kcur = kc;
   if (!C.xs_every && (left_region || T.reg(cell) != q.rxs)) {
-    q.gxs = g;
+    q.gxs = q.g;
     q.rxs = T.reg(cell);
   }
   const int xs = T.mat(cell) + C.M * q.gxs;
   Real u_acc = Num<Real>::u01(rng.next());
   if (!(u_acc * C.maj < T.sigw[xs])) return kEvStep;
+  score_segment<Real, Acc>(C, direct, diff, q, x, cell);
+  const int g = q.g;
   if (!collide<F, Real>(C, xs, rng, q, nsite)) return kEvStep | kEvReal | kEvAbsorb;
-  if (q.g != g) q.kcur = kint(C, q.g, q.x, q.cell);
+  if (q.g != g) q.ks = kint(C, q.g, x, cell);
   if (C.xs_every) {
     q.gxs = q.g;
     q.rxs = T.reg(cell);
@@ -760,7 +766,9 @@ void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim
             birth(C, p, rng, q);
             q.gxs = q.g;
             q.rxs = C.T.reg(q.cell);
-            q.kcur = kint(C, q.g, q.x, q.cell);
+            q.sx = q.x;
+            q.scell = q.cell;
+            q.ks = kint(C, q.g, q.x, q.cell);
             trace_event<F>(C, q, NRAPS_EV_BIRTH, q.cell, q.g, 0, q.x, q.mu);
             active = true;
           }

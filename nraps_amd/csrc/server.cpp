@@ -5,7 +5,7 @@
 // then runs one generation of every resident problem in the same launches
 // (the persistent event kernel visits all of them), reads back the per-slot
 // generation state in one copy, and retires requests that reached their
-// generation count or their k standard-error target.  A retired request's
+// generation count or their relative k standard-error target.  A retired request's
 // device memory is released at once, so queued work is admitted while the
 // rest keeps running.  Every request keeps its own RNG streams (problem_id),
 // so its result does not depend on what else is resident.
@@ -222,7 +222,7 @@ void nraps_server::loop() {
           double n = r->kn, mean = r->ks / n;
           double var = (r->ks2 - n * mean * mean) / (n - 1);
           double sem = var > 0 ? std::sqrt(var / n) : 0.0;
-          if (sem <= opts.sigma_k_target) finished = true;
+          if (sem <= opts.sigma_k_target * std::fabs(mean)) finished = true;
         }
         if (cancel || finished) {
           if (cs.active) r->dev->deactivate(s);

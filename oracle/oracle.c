@@ -505,6 +505,24 @@ static double okint(const oprob *p, int g, double x, int32_t c) {
   return p->kreg[g * (p->R + 1) + r] + p->nusigf[p->reg_mat[r] + p->M * g] * (x - p->reg_lo[r]);
 }
 
+/* The straight path between real events (real collisions, boundary faces,
+   the event cap) is one segment: virtual collisions change neither the
+   direction nor the group.  A segment is scored where it ends: tally over
+   its cells with weight 1/|mu|, and |K(end) - K(start)|/|mu| of k, where K
+   is the nu*Sigma_f integral from the left face. */
+typedef struct {
+  double sx;
+  int32_t scell;
+} oseg;
+
+static void oscore(const oprob *p, owork *w, oseg *s, int32_t g, double mu, double xe, int32_t ce,
+                   double *kacc) {
+  otally_piece(p, w, g, s->sx, s->scell, xe, ce, mu);
+  if (s->sx != xe) *kacc += fabs(okint(p, g, xe, ce) - okint(p, g, s->sx, s->scell)) * (1.0 / fabs(mu));
+  s->sx = xe;
+  s->scell = ce;
+}
+
 /* One history with Woodcock delta tracking against the global majorant of
    the transport cross section.  Cross-section staleness follows the
    reference (src/mc_code.rs:147): the XS group g_xs is refreshed to the
@@ -522,9 +540,11 @@ static double ohistory_woodcock(const ocycle *cy, owork *w, int64_t pidx, orng *
   obirth(cy, pidx, r, &x, &mu, &cell, &g);
   otrace(w, NRAPS_EV_BIRTH, cell, g, 0, x, mu);
   int32_t gxs = g, rxs = p->reg_of[cell];
+  oseg seg = {x, cell};
   int64_t events = 0;
   for (;;) {
     if (++events > cap) {
+      oscore(p, w, &seg, g, mu, x, cell, &kacc);
       w->st.capped++;
       otrace(w, NRAPS_EV_CAP, cell, g, 0, x, mu);
       return kacc;
@@ -532,13 +552,10 @@ static double ohistory_woodcock(const ocycle *cy, owork *w, int64_t pidx, orng *
     w->st.steps++;
     double disp = mu * -nraps_log(orng_flight(r)) * inv_maj;
     int left_region = 0;
-    double wgt = 1.0 / fabs(mu);
     for (;;) {
       double end = x + disp;
       if (mu >= 0.0 && end > p->XR) {
-        otally_piece(p, w, g, x, cell, p->XR, nc - 1, mu);
-        if (x != p->XR)
-          kacc += fabs(okint(p, g, p->XR, nc - 1) - okint(p, g, x, cell)) * wgt;
+        oscore(p, w, &seg, g, mu, p->XR, nc - 1, &kacc);
         if (p->reg_of[nc - 1] != rxs) left_region = 1;
         w->st.crossings++;
         if (p->boundr > 0.0) {
@@ -548,15 +565,13 @@ static double ohistory_woodcock(const ocycle *cy, owork *w, int64_t pidx, orng *
           disp = o3[1];
           x = o3[2];
           cell = nc - 1;
-          wgt = 1.0 / fabs(mu);
           continue;
         }
         w->st.leaks++;
         otrace(w, NRAPS_EV_LEAK, nc - 1, g, 0, p->XR, mu);
         return kacc;
       } else if (mu < 0.0 && p->XL > end) {
-        otally_piece(p, w, g, x, cell, p->XL, 0, mu);
-        if (x != p->XL) kacc += fabs(okint(p, g, p->XL, 0) - okint(p, g, x, cell)) * wgt;
+        oscore(p, w, &seg, g, mu, p->XL, 0, &kacc);
         if (p->reg_of[0] != rxs) left_region = 1;
         w->st.crossings++;
         if (p->boundl > 0.0) {
@@ -566,18 +581,16 @@ static double ohistory_woodcock(const ocycle *cy, owork *w, int64_t pidx, orng *
           disp = o3[1];
           x = o3[2];
           cell = 0;
-          wgt = 1.0 / fabs(mu);
           continue;
         }
         w->st.leaks++;
         otrace(w, NRAPS_EV_LEAK, 0, g, 0, p->XL, mu);
         return kacc;
       } else {
-        int32_t ce = end == x ? cell : olocate(p, end, mu);
-        otally_piece(p, w, g, x, cell, end, ce, mu);
-        if (end != x) kacc += fabs(okint(p, g, end, ce) - okint(p, g, x, cell)) * wgt;
-        x = end;
-        cell = ce;
+        if (end != x) {
+          cell = olocate(p, end, mu);
+          x = end;
+        }
         break;
       }
     }
@@ -588,6 +601,7 @@ static double ohistory_woodcock(const ocycle *cy, owork *w, int64_t pidx, orng *
     int xs = p->mat[cell] + M * gxs;
     double u_acc = orng_u(r);
     if (u_acc * p->maj < p->sigw[xs]) {
+      oscore(p, w, &seg, g, mu, x, cell, &kacc);
       if (!ocollide(cy, w, r, pidx, xs, cell, x, &g, &mu)) return kacc;
       if (xs_every) {
         gxs = g;

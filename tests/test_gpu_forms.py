@@ -87,7 +87,8 @@ def test_server_sigma_stop(gpu):
         t1 = srv.submit(p, o)
         r1 = srv.wait(t1, timeout_ms=120000)
         st, q, tot, cycles = srv.latency(t1)
-        assert r1.status_code == 0 and 3 + 5 <= cycles < 400 and r1.k_sigma <= 5e-3
+        assert r1.status_code == 0 and 3 + 5 <= cycles < 400
+        assert r1.k_sigma <= 5e-3 * r1.k_mean
         assert r1.histories_run == cycles * o.histories
 
 
@@ -217,3 +218,27 @@ def test_full_size_c1_invariants(gpu):
     o.precision = _abi.FP64
     r64 = nr.solve(p, o)
     assert abs(r64.k_mean - r.k_mean) < 5 * math.hypot(r.k_sigma, r64.k_sigma)
+
+
+def test_fp32_tally_reduction_error(gpu, monkeypatch):
+    """Error of the FP32 tally reduction, isolated: the same FP32 trajectories
+    scored (a) as FP32 partials in each block's LDS, flushed into the FP64
+    generation tally (the production path) and (b) straight into the FP64
+    tally (LDS budgets forced to 0).  k is FP64 per history either way and
+    must be bit-identical; the last generation's tally agrees to 1e-5 of
+    each group's largest bin, and so do the active-mean fluxes."""
+    p, _ = nr.generate(0)
+    kw = dict(histories=10**6, generations=4, skip=1, method=_abi.EVENT_WOODCOCK,
+              precision=_abi.FP32, use_graph=0)
+    a = nr.solve(p, nr.run_opts(**kw))
+    monkeypatch.setenv("NRAPS_LDS_SMALL", "0")
+    monkeypatch.setenv("NRAPS_LDS_BUDGET", "0")
+    b = nr.solve(p, nr.run_opts(**kw))
+    assert_same_k(a, b)
+    G = p.n_groups
+    ta, tb = a.tally_last.reshape(G, -1), b.tally_last.reshape(G, -1)
+    err = np.abs(ta - tb).max(axis=1) / tb.max(axis=1)
+    print("FP32 LDS tally vs FP64 tally, max error / group max:", err)
+    assert np.all(err < 1e-5)
+    fa, fb = a.flux_mean.reshape(G, -1), b.flux_mean.reshape(G, -1)
+    assert np.all(np.abs(fa - fb).max(axis=1) / fb.max(axis=1) < 1e-5)

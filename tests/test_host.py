@@ -5,6 +5,7 @@ output files, the seeded workload generators, validation, and the ABI.
 """
 import ctypes as C
 import json
+import math
 import os
 import re
 
@@ -219,3 +220,16 @@ def test_no_gpu_means_an_error_not_a_fallback():
     rb = nr.solve(p, o, check=False)
     assert rb.status_code == _abi.NRAPS_E_DEVICE
     assert "device" in nr.last_error()
+
+
+def test_serving_arrivals_use_pcg32():
+    """The serving benchmark's arrival generator is PCG32 (pcg-c-basic KAT
+    srandom(42, 54)) and gives Poisson arrivals at the requested rate."""
+    from nraps_amd.serve_bench import Pcg32, arrival_times, percentile
+    g = Pcg32(42, 54)
+    assert [g.next() for _ in range(6)] == [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293,
+                                            0xbfa4784b, 0xcbed606e]
+    t = arrival_times(200.0, 60.0)
+    assert abs(len(t) - 12000) < 5 * math.sqrt(12000)
+    assert all(0 <= a < b < 60.0 for a, b in zip(t, t[1:]))
+    assert percentile([3.0, 1.0, 2.0], 0.5) == 2.0
