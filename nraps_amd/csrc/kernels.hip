@@ -205,6 +205,10 @@ struct Particle {
   Real sx;
   int32_t scell;
   double ks;
+  // XS of the particle while it stays in region rxs: index (mat + M*gxs) and
+  // the Woodcock acceptance XS there
+  int32_t xs;
+  Real sw;
   int32_t nsites, tr;
   bool traced;
 };
@@ -330,17 +334,17 @@ struct SiteCursor {
   }
 };
 
-// Collision at (q.x, q.cell), src/mc_code.rs:113-132: draws U_int, U_mu,
+// Collision at (q.x, q.cell) in material m, src/mc_code.rs:113-132: draws U_int, U_mu,
 // U_scat; the scatter CDF uses the row of the current group and the
 // normaliser of the (possibly stale) XS index; the number of fission sites
 // is drawn after the reference's three draws and returned in `nsite` (the
 // caller banks them at the collision site).  Returns true if the particle
 // scattered.
 template <class F, typename Real>
-__device__ inline bool collide(const Ctx<Real> &C, int xs, LaneRng &rng, Particle<Real> &q,
+__device__ inline bool collide(const Ctx<Real> &C, int xs, int m, LaneRng &rng, Particle<Real> &q,
                                int &nsite) {
   const Tabs<Real> &T = C.T;
-  const int G = C.G, m = T.mat(q.cell);
+  const int G = C.G;
   Real u_int = Num<Real>::u01(rng.next());
   Real mu_new = Real(2) * Num<Real>::u01(rng.next()) - Real(1);
   Real u_scat = Num<Real>::u01(rng.next());
@@ -471,7 +475,7 @@ __device__ void history_surface(const Ctx<double> &C, const DevProblem &P, int64
         q.x = end_x;
         st.v[kStCollisions]++;
         int nsite;
-        const bool scattered = collide<FeatAll, double>(C, xs, rng, q, nsite);
+        const bool scattered = collide<FeatAll, double>(C, xs, mc, rng, q, nsite);
         if (nsite) emit_sites_plain(C, nsite, q);
         if (!scattered) {
           st.v[kStAbsorb]++;
@@ -555,6 +559,16 @@ __device__ inline double kint(const Ctx<Real> &C, int g, Real x, int c) {
   return T.kreg[g * (C.R + 1) + r] + T.nsfd[T.mat(c) + C.M * g] * ((double)x - T.reglo[r]);
 }
 
+// XS refresh: the particle's XS group gxs is taken in region r (material of
+// `cell`).  Staleness follows src/mc_code.rs:147: with the reference rule the
+// caller sets gxs = g only on entering another region.
+template <typename Real>
+__device__ inline void set_xs(const Ctx<Real> &C, Particle<Real> &q, int r, int cell) {
+  q.rxs = r;
+  q.xs = C.T.mat(cell) + C.M * q.gxs;
+  q.sw = C.T.sigw[q.xs];
+}
+
 // Close the straight segment that started at (q.sx, q.scell) at (xe, ce):
 // its track-length tally and its k contribution, both with weight 1/|mu|.
 // The next segment starts there with k integral `kb`.
@@ -631,20 +645,23 @@ __device__ inline uint32_t woodcock_step(const Ctx<Real> &C, Acc *direct, Acc *d
   }
   q.x = x;
   q.cell = cell;
-  if (!C.xs_every && (left_region || T.reg(cell) != q.rxs)) {
-    q.gxs = q.g;
-    q.rxs = T.reg(cell);
+  const int r = T.reg(cell);
+  if (left_region || r != q.rxs) {
+    if (!C.xs_every) q.gxs = q.g;
+    set_xs(C, q, r, cell);
   }
-  const int xs = T.mat(cell) + C.M * q.gxs;
   Real u_acc = Num<Real>::u01(rng.next());
-  if (!(u_acc * C.maj < T.sigw[xs])) return kEvStep;
+  if (!(u_acc * C.maj < q.sw)) return kEvStep;
   score_segment<Real, Acc>(C, direct, diff, q, x, cell);
   const int g = q.g;
-  if (!collide<F, Real>(C, xs, rng, q, nsite)) return kEvStep | kEvReal | kEvAbsorb;
-  if (q.g != g) q.ks = kint(C, q.g, x, cell);
-  if (C.xs_every) {
-    q.gxs = q.g;
-    q.rxs = T.reg(cell);
+  if (!collide<F, Real>(C, q.xs, q.xs - C.M * q.gxs, rng, q, nsite))
+    return kEvStep | kEvReal | kEvAbsorb;
+  if (q.g != g) {
+    q.ks = kint(C, q.g, x, cell);
+    if (C.xs_every) {
+      q.gxs = q.g;
+      set_xs(C, q, r, cell);
+    }
   }
   return kEvStep | kEvReal;
 }
@@ -765,7 +782,7 @@ void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim
             init_particle<F>(C, p, rng, q);
             birth(C, p, rng, q);
             q.gxs = q.g;
-            q.rxs = C.T.reg(q.cell);
+            set_xs(C, q, C.T.reg(q.cell), q.cell);
             q.sx = q.x;
             q.scell = q.cell;
             q.ks = kint(C, q.g, q.x, q.cell);
