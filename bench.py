@@ -9,7 +9,8 @@ A step is one full k-eigenvalue solve of that workload (120 generations).
 `value` is the FP32 event-based (Woodcock) histories/s over the K timed
 steps; the plain FP64 port (one thread per history, surface tracking) runs
 the same workload once beside it, and the FP64 CPU oracle is timed on a
-bounded sample on the host cores.
+bounded sample on the host cores.  The serving form runs a 10 s Poisson
+request stream (50 req/s) beside them for the p50/p99 solve latency.
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -41,6 +42,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the plain FP64 port leg")
     ap.add_argument("--cpu-histories", type=int, default=10_000_000)
     ap.add_argument("--cpu-generations", type=int, default=24)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving latency leg")
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -115,6 +119,17 @@ def main():
         out["k_fp64_plain_sigma"] = r64.k_sigma
         out["k_delta_over_sigma"] = abs(k32 - r64.k_mean) / comb if comb > 0 else None
         log(f"plain FP64 port: {v64:.4g} histories/s, k = {r64.k_mean:.6f} +- {r64.k_sigma:.6f}")
+
+    if not args.no_serve:
+        # serving form: Poisson requests of uneven slab problems (fission bank,
+        # stop at 50 pcm relative k standard error), continuous batching
+        from nraps_amd.serve_bench import run_serving
+        sv = run_serving(args.serve_rate, args.serve_seconds, drain_s=300.0, log=log)
+        out["latency"] = {k: sv[k] for k in ("rate_per_s", "seconds", "submitted", "completed",
+                                             "p50_ms", "p99_ms", "histories_per_s",
+                                             "sigma_k_target_pcm")}
+        log(f"serving at {args.serve_rate:g} req/s: p50 {sv['p50_ms']:.1f} ms, "
+            f"p99 {sv['p99_ms']:.1f} ms")
 
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))

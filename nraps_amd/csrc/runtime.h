@@ -23,13 +23,17 @@ struct DeviceError : std::runtime_error {
     if (e_ != hipSuccess) throw ::nraps::DeviceError(#x, e_); \
   } while (0)
 
-// Owns device allocations; frees them on destruction.
+// Owns device allocations; frees them on destruction.  Bound to a stream,
+// it allocates from the device's stream-ordered memory pool on that stream
+// (no device-wide synchronisation per allocation or free); the stream must
+// outlive the arena.
 class DevArena {
  public:
   DevArena() = default;
   DevArena(const DevArena &) = delete;
   DevArena &operator=(const DevArena &) = delete;
   ~DevArena();
+  void bind(hipStream_t s) { stream_ = s; pooled_ = true; }
   void *raw(size_t bytes);
   template <class T>
   T *alloc(size_t n) {
@@ -40,7 +44,13 @@ class DevArena {
  private:
   std::vector<void *> ptrs_;
   size_t bytes_ = 0;
+  hipStream_t stream_ = nullptr;
+  bool pooled_ = false;
 };
+
+// Keep freed stream-ordered allocations of `device` cached in its default
+// memory pool (set once per process and device).
+void keep_pool_memory(int device);
 
 int ktree_levels(int64_t n);
 int64_t site_capacity(int64_t N);
@@ -50,10 +60,21 @@ int64_t problem_footprint(const nraps_problem *pb, const nraps_run_opts *o);
 // One problem's device state: tables, banks, tallies, accumulators.
 class ProblemDev {
  public:
+  // Host half of the set-up: derived tables and the packed blob (no HIP).
+  void prepare(const nraps_problem *pb, const nraps_run_opts &o, bool fp32);
+  // Device half: allocations and uploads, asynchronous on `s` when it is
+  // not null (host sources are owned by this object or by the caller's
+  // problem, which must stay alive until the stream has run them).
   // cs_slot: externally owned CycleState (server slot array) or nullptr.
+  void upload(bool woodcock, CycleState *cs_slot, bool want_trace, nraps_result *trace_res,
+              hipStream_t s);
   void setup(const nraps_problem *pb, const nraps_run_opts &o, bool fp32, bool woodcock,
-             CycleState *cs_slot, bool want_trace, nraps_result *trace_res);
-  // Copy results into r; returns the status of the run.
+             CycleState *cs_slot, bool want_trace, nraps_result *trace_res, hipStream_t s) {
+    prepare(pb, o, fp32);
+    upload(woodcock, cs_slot, want_trace, trace_res, s);
+  }
+  // Copy results into r (one synchronisation of the set-up stream); returns
+  // the status of the run.
   int download(nraps_result *r, double ms_total, double ms_device, double ms_transport,
                int64_t launches);
   // Stop the problem from taking part in further launches.
@@ -64,8 +85,11 @@ class ProblemDev {
   Prepared prep;
   std::vector<uint8_t> blob;
   TabLayout lay{};
-  bool fission = false;
+  bool fission = false, fp32 = false;
   int64_t trace_n = 0;
+  std::vector<double> width;
+  CycleState cs0{};
+  hipStream_t stream = nullptr;
   DevProblem dev{};
   DevFinal fin{};
   CycleState *d_cs = nullptr;
@@ -81,7 +105,9 @@ class Launcher {
  public:
   explicit Launcher(int device);
   ~Launcher();
-  void build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodcock);
+  // Launch plan for a set of problems; the tables go to the device on `s`
+  // (ordered after the work already queued there) when it is not null.
+  void build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodcock, hipStream_t s = nullptr);
   // Launches of one generation for every problem of the set.
   void enqueue_cycle(hipStream_t s, hipEvent_t tr0, hipEvent_t tr1);
   // Capture one generation into a graph (replaces any previous capture).
@@ -100,6 +126,9 @@ class Launcher {
   size_t lds_bytes_ = 0;
   bool fp32_ = false, woodcock_ = false, fission_any_ = false, lds_tally_ = false;
   WcVariant variant_{};
+  std::vector<DevProblem> h_probs_;
+  std::vector<DevFinal> h_fins_;
+  std::vector<int64_t> h_tile_base_;
   DevProblem *d_probs_ = nullptr;
   DevFinal *d_fins_ = nullptr;
   int64_t *d_tile_sums_ = nullptr, *d_tile_base_ = nullptr;

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -27,17 +28,30 @@ DeviceError::DeviceError(const char *what, hipError_t e)
 #define HIP_TRY NRAPS_HIP_TRY
 
 DevArena::~DevArena() {
-  for (void *p : ptrs_) (void)hipFree(p);
+  for (void *p : ptrs_) (void)(pooled_ ? hipFreeAsync(p, stream_) : hipFree(p));
 }
 
 void *DevArena::raw(size_t bytes) {
   if (bytes == 0) bytes = 16;
+  bytes = (bytes + 255) & ~(size_t)255;
   void *p = nullptr;
-  hipError_t e = hipMalloc(&p, bytes);
-  if (e != hipSuccess) throw DeviceError("hipMalloc", e);
+  hipError_t e = pooled_ ? hipMallocAsync(&p, bytes, stream_) : hipMalloc(&p, bytes);
+  if (e != hipSuccess) throw DeviceError(pooled_ ? "hipMallocAsync" : "hipMalloc", e);
   ptrs_.push_back(p);
   bytes_ += bytes;
   return p;
+}
+
+void keep_pool_memory(int device) {
+  static std::mutex mu;
+  static std::vector<int> done;
+  std::lock_guard<std::mutex> g(mu);
+  if (std::find(done.begin(), done.end(), device) != done.end()) return;
+  hipMemPool_t pool;
+  NRAPS_HIP_TRY(hipDeviceGetDefaultMemPool(&pool, device));
+  uint64_t keep = UINT64_MAX;
+  NRAPS_HIP_TRY(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  done.push_back(device);
 }
 
 static int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -84,20 +98,35 @@ int64_t problem_footprint(const nraps_problem *pb, const nraps_run_opts *o) {
 // ---------------------------------------------------------------------------
 // ProblemDev
 // ---------------------------------------------------------------------------
-void ProblemDev::setup(const nraps_problem *pb_, const nraps_run_opts &o_, bool fp32, bool woodcock,
-                       CycleState *cs_slot, bool want_trace, nraps_result *r) {
+void ProblemDev::prepare(const nraps_problem *pb_, const nraps_run_opts &o_, bool fp32_) {
   pb = pb_;
   o = o_;
+  fp32 = fp32_;
   prep.build(pb);
   blob = prep.pack(fp32, &lay);
   fission = o.source == NRAPS_SOURCE_FISSION_BANK;
+  width.resize(prep.nc);
+  for (int c = 0; c < prep.nc; c++) width[c] = pb->mesh_right[c] - pb->mesh_left[c];
+}
+
+void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nraps_result *r,
+                        hipStream_t s) {
+  stream = s;
+  if (s) arena.bind(s);
+  auto h2d = [&](void *dst, const void *src, size_t n) {
+    HIP_TRY(s ? hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s)
+              : hipMemcpy(dst, src, n, hipMemcpyHostToDevice));
+  };
+  auto zero = [&](void *dst, size_t n) {
+    HIP_TRY(s ? hipMemsetAsync(dst, 0, n, s) : hipMemset(dst, 0, n));
+  };
   const Prepared &P = prep;
   const int64_t N = o.histories;
   const int rs = fp32 ? 4 : 8;
   DevProblem &d = dev;
   std::memset(&d, 0, sizeof d);
   uint8_t *tab = arena.alloc<uint8_t>(blob.size());
-  HIP_TRY(hipMemcpy(tab, blob.data(), blob.size(), hipMemcpyHostToDevice));
+  h2d(tab, blob.data(), blob.size());
   d.tab = tab;
   d.lay = lay;
   d.nc = P.nc;
@@ -120,12 +149,11 @@ void ProblemDev::setup(const nraps_problem *pb_, const nraps_run_opts &o_, bool 
   d.XR = P.XR;
   d.maj = P.maj;
   d.inv_maj = 1.0 / P.maj;
-  CycleState cs;
-  std::memset(&cs, 0, sizeof cs);
-  cs.k_prev = o.k_initial;
-  cs.active = 1;
+  std::memset(&cs0, 0, sizeof cs0);
+  cs0.k_prev = o.k_initial;
+  cs0.active = 1;
   d_cs = cs_slot ? cs_slot : arena.alloc<CycleState>(1);
-  HIP_TRY(hipMemcpy(d_cs, &cs, sizeof cs, hipMemcpyHostToDevice));
+  h2d(d_cs, &cs0, sizeof cs0);
   d.cs = d_cs;
   d.k = arena.alloc<double>(o.generations);
   d.kacc = arena.alloc<double>(N);
@@ -151,24 +179,22 @@ void ProblemDev::setup(const nraps_problem *pb_, const nraps_run_opts &o_, bool 
     d.trace_len = arena.alloc<int32_t>(trace_n);
     d.draws = arena.alloc<int64_t>(trace_n);
     d.trace_kacc = arena.alloc<double>(trace_n);
-    HIP_TRY(hipMemset(d.trace, 0, sizeof(nraps_trace_event) * trace_n * o.trace_events));
-    HIP_TRY(hipMemset(d.trace_len, 0, sizeof(int32_t) * trace_n));
+    zero(d.trace, sizeof(nraps_trace_event) * trace_n * o.trace_events);
+    zero(d.trace_len, sizeof(int32_t) * trace_n);
     d.trace_n = trace_n;
     d.trace_ev = o.trace_events;
     d.trace_cycle = o.trace_cycle;
   }
   d_stats = arena.alloc<unsigned long long>(kNumStats);
-  HIP_TRY(hipMemset(d_stats, 0, sizeof(unsigned long long) * kNumStats));
+  zero(d_stats, sizeof(unsigned long long) * kNumStats);
   d.stats = d_stats;
 
   DevFinal &f = fin;
   std::memset(&f, 0, sizeof f);
   const int nc = P.nc, GM = P.G * P.M, Gnc = P.G * P.nc;
-  std::vector<double> width(nc);
-  for (int c = 0; c < nc; c++) width[c] = pb->mesh_right[c] - pb->mesh_left[c];
   auto up_d = [&](const double *src, size_t cnt) {
     double *p = arena.alloc<double>(cnt);
-    HIP_TRY(hipMemcpy(p, src, cnt * 8, hipMemcpyHostToDevice));
+    h2d(p, src, cnt * 8);
     return p;
   };
   f.dx = up_d(pb->delta_x, nc);
@@ -176,7 +202,7 @@ void ProblemDev::setup(const nraps_problem *pb_, const nraps_run_opts &o_, bool 
   f.nut = up_d(pb->nut, GM);
   f.sigf = up_d(pb->sigf, GM);
   int32_t *mat = arena.alloc<int32_t>(nc);
-  HIP_TRY(hipMemcpy(mat, pb->matid, nc * 4, hipMemcpyHostToDevice));
+  h2d(mat, pb->matid, nc * 4);
   f.mat = mat;
   f.nut_conv = P.nut_conv;
   f.length = P.XR;
@@ -190,14 +216,14 @@ void ProblemDev::setup(const nraps_problem *pb_, const nraps_run_opts &o_, bool 
   f.fsq = d_fsq = arena.alloc<double>(Gnc);
   f.k_ref = d_kref = arena.alloc<double>(o.generations);
   f.sites_hist = d_sites = arena.alloc<int64_t>(o.generations);
-  HIP_TRY(hipMemset(d_flux_acc, 0, Gnc * 8));
-  HIP_TRY(hipMemset(d_fs_acc, 0, nc * 8));
-  HIP_TRY(hipMemset(d_fsum, 0, Gnc * 8));
-  HIP_TRY(hipMemset(d_fsq, 0, Gnc * 8));
-  HIP_TRY(hipMemset(d_tally_last, 0, Gnc * 8));
-  HIP_TRY(hipMemset(d.k, 0, o.generations * 8));
-  HIP_TRY(hipMemset(d_kref, 0, o.generations * 8));
-  HIP_TRY(hipMemset(d_sites, 0, o.generations * 8));
+  zero(d_flux_acc, Gnc * 8);
+  zero(d_fs_acc, nc * 8);
+  zero(d_fsum, Gnc * 8);
+  zero(d_fsq, Gnc * 8);
+  zero(d_tally_last, Gnc * 8);
+  zero(d.k, o.generations * 8);
+  zero(d_kref, o.generations * 8);
+  zero(d_sites, o.generations * 8);
   f.has_diff = woodcock ? 1 : 0;
 }
 
@@ -211,23 +237,36 @@ int ProblemDev::download(nraps_result *r, double ms_total, double ms_device, dou
                          int64_t launches) {
   const Prepared &P = prep;
   const int nc = P.nc, G = P.G, gens = o.generations, Gnc = G * nc;
+  // every copy is queued on the set-up stream (after the work of the run)
+  // and waited for once
+  auto d2h = [&](void *dst, const void *src, size_t n) {
+    HIP_TRY(stream ? hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, stream)
+                   : hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
+  };
   CycleState cs;
-  HIP_TRY(hipMemcpy(&cs, d_cs, sizeof cs, hipMemcpyDeviceToHost));
-  // generations actually run (the server may stop a problem early)
-  const int ran = std::min(gens, (int)cs.cycle);
-  const int skip = o.skip;
   std::vector<double> k(gens), kref(gens), flux(Gnc), fs(nc), fsum(Gnc), fsq(Gnc), tl(Gnc);
   std::vector<int64_t> sites(gens);
   unsigned long long st[kNumStats];
-  HIP_TRY(hipMemcpy(k.data(), dev.k, gens * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(kref.data(), d_kref, gens * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(sites.data(), d_sites, gens * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(flux.data(), d_flux_acc, Gnc * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(fs.data(), d_fs_acc, nc * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(fsum.data(), d_fsum, Gnc * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(fsq.data(), d_fsq, Gnc * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(tl.data(), d_tally_last, Gnc * 8, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(st, d_stats, sizeof st, hipMemcpyDeviceToHost));
+  d2h(&cs, d_cs, sizeof cs);
+  d2h(k.data(), dev.k, gens * 8);
+  d2h(kref.data(), d_kref, gens * 8);
+  d2h(sites.data(), d_sites, gens * 8);
+  d2h(flux.data(), d_flux_acc, Gnc * 8);
+  d2h(fs.data(), d_fs_acc, nc * 8);
+  d2h(fsum.data(), d_fsum, Gnc * 8);
+  d2h(fsq.data(), d_fsq, Gnc * 8);
+  d2h(tl.data(), d_tally_last, Gnc * 8);
+  d2h(st, d_stats, sizeof st);
+  if (r && trace_n > 0) {
+    d2h(r->trace, dev.trace, sizeof(nraps_trace_event) * trace_n * o.trace_events);
+    d2h(r->trace_len, dev.trace_len, 4 * trace_n);
+    d2h(r->draws, dev.draws, 8 * trace_n);
+    d2h(r->kacc, dev.trace_kacc, 8 * trace_n);
+  }
+  if (stream) HIP_TRY(hipStreamSynchronize(stream));
+  // generations actually run (the server may stop a problem early)
+  const int ran = std::min(gens, (int)cs.cycle);
+  const int skip = o.skip;
   if (!r) return NRAPS_OK;
   auto put = [](double *dst, const std::vector<double> &v) {
     if (dst) std::memcpy(dst, v.data(), v.size() * 8);
@@ -288,13 +327,6 @@ int ProblemDev::download(nraps_result *r, double ms_total, double ms_device, dou
   r->ms_device = ms_device;
   r->ms_transport = ms_transport;
   r->transport_launches = launches;
-  if (trace_n > 0) {
-    HIP_TRY(hipMemcpy(r->trace, dev.trace, sizeof(nraps_trace_event) * trace_n * o.trace_events,
-                      hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(r->trace_len, dev.trace_len, 4 * trace_n, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(r->draws, dev.draws, 8 * trace_n, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy(r->kacc, dev.trace_kacc, 8 * trace_n, hipMemcpyDeviceToHost));
-  }
   int status = NRAPS_OK;
   if (cs.error & 1)
     status = NRAPS_E_BANK_OVERFLOW;
@@ -331,7 +363,7 @@ void Launcher::drop_graph() {
   graph_ = nullptr;
 }
 
-void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodcock) {
+void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodcock, hipStream_t s) {
   drop_graph();
   n_ = (int)ps.size();
   fp32_ = fp32;
@@ -429,9 +461,21 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     df[i] = ps[i]->fin;
   }
   if (n_ > 0) {
-    HIP_TRY(hipMemcpy(d_probs_, dp.data(), sizeof(DevProblem) * n_, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_fins_, df.data(), sizeof(DevFinal) * n_, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d_tile_base_, tile_base.data(), sizeof(int64_t) * n_, hipMemcpyHostToDevice));
+    // staged copies: the host vectors die with this call, so the async path
+    // keeps them in the launcher until the next build
+    h_probs_ = std::move(dp);
+    h_fins_ = std::move(df);
+    h_tile_base_ = std::move(tile_base);
+    if (s) {
+      HIP_TRY(hipMemcpyAsync(d_probs_, h_probs_.data(), sizeof(DevProblem) * n_, hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(d_fins_, h_fins_.data(), sizeof(DevFinal) * n_, hipMemcpyHostToDevice, s));
+      HIP_TRY(hipMemcpyAsync(d_tile_base_, h_tile_base_.data(), sizeof(int64_t) * n_,
+                             hipMemcpyHostToDevice, s));
+    } else {
+      HIP_TRY(hipMemcpy(d_probs_, h_probs_.data(), sizeof(DevProblem) * n_, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d_fins_, h_fins_.data(), sizeof(DevFinal) * n_, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d_tile_base_, h_tile_base_.data(), sizeof(int64_t) * n_, hipMemcpyHostToDevice));
+    }
   }
 }
 
@@ -517,19 +561,21 @@ int solve_batch(int n, const nraps_problem *const *pbs, const nraps_run_opts *op
     const bool fp32 = opts[0].precision == NRAPS_FP32;
     const bool woodcock = opts[0].method == NRAPS_EVENT_WOODCOCK;
     const bool use_graph = opts[0].use_graph != 0;
+    // the stream outlives the problems: their memory is freed on it
+    StreamGuard sg;
+    HIP_TRY(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
     std::vector<std::unique_ptr<ProblemDev>> owned(n);
     std::vector<ProblemDev *> ps(n);
     int gens_max = 0;
     for (int i = 0; i < n; i++) {
       owned[i].reset(new ProblemDev());
-      owned[i]->setup(pbs[i], opts[i], fp32, woodcock, nullptr, true, results ? results[i] : nullptr);
+      owned[i]->setup(pbs[i], opts[i], fp32, woodcock, nullptr, true, results ? results[i] : nullptr,
+                      sg.s);
       ps[i] = owned[i].get();
       gens_max = std::max(gens_max, (int)opts[i].generations);
     }
     Launcher L(device);
-    L.build(ps, fp32, woodcock);
-    StreamGuard sg;
-    HIP_TRY(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking));
+    L.build(ps, fp32, woodcock, sg.s);
     hipEvent_t e0 = sg.make(), e1 = sg.make();
     std::vector<hipEvent_t> tr;
     HIP_TRY(hipEventRecord(e0, sg.s));

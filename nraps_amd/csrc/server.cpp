@@ -81,7 +81,7 @@ struct nraps_server {
 };
 
 void nraps_server::retire(Request *r, int status) {
-  // owner thread: copy results out and release the device memory
+  // owner thread, stream idle: copy results out and release the device memory
   if (r->dev) {
     try {
       if (status != NRAPS_E_CANCELLED) {
@@ -116,7 +116,7 @@ void nraps_server::retire(Request *r, int status) {
       status = NRAPS_E_DEVICE;
       set_error(e.what());
     }
-    r->dev.reset();
+    r->dev.reset();  // memory returns to the stream-ordered pool
   }
   std::lock_guard<std::mutex> g(m);
   if (r->slot >= 0) free_slots.push_back(r->slot);
@@ -130,20 +130,34 @@ void nraps_server::retire(Request *r, int status) {
   cv_done.notify_all();
 }
 
+// One iteration of the owner loop:
+//   1. admit queued requests (first fit by memory footprint and free slots)
+//      and build their host tables -- while the GPU runs the cycle queued by
+//      the previous iteration;
+//   2. wait for that cycle, read every slot's generation state, retire the
+//      requests that finished (or were cancelled);
+//   3. queue the device set-up of the admitted requests, rebuild the launch
+//      plan if membership changed, and queue the next cycle of every
+//      resident request together with the read-back of the slot states.
+// All device work is ordered on one stream; allocations come from the
+// stream-ordered pool, so set-up and retirement never stall the device.
 void nraps_server::loop() {
   try {
     NRAPS_HIP_TRY(hipSetDevice(opts.device));
     NRAPS_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    keep_pool_memory(opts.device);
     L.reset(new Launcher(opts.device));
     bool dirty = true;
     const bool fp32 = opts.precision == NRAPS_FP32;
+    std::vector<Request *> inflight;  // requests of the queued cycle
     for (;;) {
       std::vector<Request *> admit;
       {
         std::unique_lock<std::mutex> lk(m);
-        cv_work.wait(lk, [&] { return stop || !pending.empty() || !resident.empty(); });
+        cv_work.wait(lk, [&] {
+          return stop || !pending.empty() || !resident.empty() || !inflight.empty();
+        });
         if (stop) break;
-        // first-fit admission in arrival order
         for (auto it = pending.begin(); it != pending.end();) {
           Request *r = *it;
           if (r->cancel) {
@@ -167,16 +181,64 @@ void nraps_server::loop() {
           }
         }
       }
+      // host half of the set-up, overlapping the queued cycle
+      std::vector<Request *> ready;
       for (Request *r : admit) {
         r->t_admit = Clock::now();
         try {
           r->dev.reset(new ProblemDev());
-          r->dev->setup(&r->pb.view, r->o, fp32, true, &d_slots[r->slot], false, nullptr);
+          r->dev->prepare(&r->pb.view, r->o, fp32);
+          ready.push_back(r);
+        } catch (const std::exception &e) {
+          set_error(e.what());
+          r->dev.reset();
+          retire(r, NRAPS_E_INVALID);
+        }
+      }
+      // the queued cycle: wait, then read the generation state of its slots
+      if (!inflight.empty()) {
+        NRAPS_HIP_TRY(hipStreamSynchronize(s));
+        for (Request *r : inflight) {
+          const CycleState &cs = h_slots[r->slot];
+          if (cs.cycle > r->cycles_run) {
+            const int done_gen = cs.cycle - 1;  // generation just closed
+            if (done_gen >= r->o.skip) {
+              r->ks += cs.k_prev;
+              r->ks2 += cs.k_prev * cs.k_prev;
+              r->kn++;
+            }
+            r->cycles_run = cs.cycle;
+          }
+          bool cancel;
+          {
+            std::lock_guard<std::mutex> g(m);
+            cancel = r->cancel;
+          }
+          bool finished = !cs.active;
+          if (!finished && opts.sigma_k_target > 0 && r->kn >= std::max(2, opts.min_active)) {
+            double n = r->kn, mean = r->ks / n;
+            double var = (r->ks2 - n * mean * mean) / (n - 1);
+            double sem = var > 0 ? std::sqrt(var / n) : 0.0;
+            if (sem <= opts.sigma_k_target * std::fabs(mean)) finished = true;
+          }
+          if (cancel || finished) {
+            if (cs.active) r->dev->deactivate(s);
+            retire(r, cancel ? NRAPS_E_CANCELLED : NRAPS_OK);
+            dirty = true;
+          }
+        }
+        inflight.clear();
+      }
+      // device half of the set-up, queued on the stream
+      for (Request *r : ready) {
+        try {
+          r->dev->upload(true, &d_slots[r->slot], false, nullptr, s);
           std::lock_guard<std::mutex> g(m);
           resident.push_back(r);
           dirty = true;
         } catch (const std::exception &e) {
           set_error(e.what());
+          NRAPS_HIP_TRY(hipStreamSynchronize(s));
           r->dev.reset();
           retire(r, NRAPS_E_NOMEM);
         }
@@ -190,7 +252,7 @@ void nraps_server::loop() {
       if (dirty) {
         std::vector<ProblemDev *> ps;
         for (Request *r : run) ps.push_back(r->dev.get());
-        L->build(ps, fp32, true);
+        L->build(ps, fp32, true, s);
         if (opts.use_graph) L->capture(s);
         dirty = false;
       }
@@ -200,36 +262,7 @@ void nraps_server::loop() {
         L->enqueue_cycle(s, nullptr, nullptr);
       NRAPS_HIP_TRY(hipMemcpyAsync(h_slots.data(), d_slots, sizeof(CycleState) * h_slots.size(),
                                    hipMemcpyDeviceToHost, s));
-      NRAPS_HIP_TRY(hipStreamSynchronize(s));
-      for (Request *r : run) {
-        const CycleState &cs = h_slots[r->slot];
-        if (cs.cycle > r->cycles_run) {
-          const int done_gen = cs.cycle - 1;  // generation just closed
-          if (done_gen >= r->o.skip) {
-            r->ks += cs.k_prev;
-            r->ks2 += cs.k_prev * cs.k_prev;
-            r->kn++;
-          }
-          r->cycles_run = cs.cycle;
-        }
-        bool cancel;
-        {
-          std::lock_guard<std::mutex> g(m);
-          cancel = r->cancel;
-        }
-        bool finished = !cs.active;
-        if (!finished && opts.sigma_k_target > 0 && r->kn >= std::max(2, opts.min_active)) {
-          double n = r->kn, mean = r->ks / n;
-          double var = (r->ks2 - n * mean * mean) / (n - 1);
-          double sem = var > 0 ? std::sqrt(var / n) : 0.0;
-          if (sem <= opts.sigma_k_target * std::fabs(mean)) finished = true;
-        }
-        if (cancel || finished) {
-          if (cs.active) r->dev->deactivate(s);
-          retire(r, cancel ? NRAPS_E_CANCELLED : NRAPS_OK);
-          dirty = true;
-        }
-      }
+      inflight = run;
     }
   } catch (const std::exception &e) {
     std::lock_guard<std::mutex> g(m);
@@ -243,18 +276,31 @@ void nraps_server::loop() {
       }
     }
     pending.clear();
-    resident.clear();
     cv_done.notify_all();
   }
-  // release device memory of anything still resident
+  // release device memory of anything still resident (while the stream its
+  // pool allocations are ordered on is alive)
+  if (s) (void)hipStreamSynchronize(s);
   std::vector<Request *> left;
   {
     std::lock_guard<std::mutex> g(m);
     left = resident;
   }
-  for (Request *r : left) retire(r, NRAPS_E_CANCELLED);
+  const bool failed = !fatal.empty();
+  for (Request *r : left) {
+    if (failed) {
+      r->dev.reset();
+      std::lock_guard<std::mutex> g(m);
+      resident.erase(std::remove(resident.begin(), resident.end(), r), resident.end());
+    } else {
+      retire(r, NRAPS_E_CANCELLED);
+    }
+  }
   L.reset();
-  if (s) (void)hipStreamDestroy(s);
+  if (s) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+  }
   s = nullptr;
 }
 
