@@ -215,11 +215,14 @@ struct Particle {
 
 // First index whose CDF value is not below u, clamped to the last group:
 // slice.partition_point(|&x| x < u).min(n - 1), src/mc_code.rs:30-31, :124-126.
+// A cumulative table is non-decreasing, so that index is the number of the
+// first n - 1 entries below u: counted with a fixed trip count (n is uniform
+// across the wave), no divergent early exit.
 template <typename Real>
 __device__ inline int cdf_index(const Real *cdf, int n, Real u) {
   int i = 0;
-  while (i < n && cdf[i] < u) i++;
-  return i < n - 1 ? i : n - 1;
+  for (int j = 0; j < n - 1; j++) i += cdf[j] < u ? 1 : 0;
+  return i;
 }
 
 template <class F, typename Real>
@@ -360,15 +363,12 @@ __device__ inline bool collide(const Ctx<Real> &C, int xs, int m, LaneRng &rng, 
     trace_event<F>(C, q, NRAPS_EV_ABSORB, q.cell, q.g, q.g, q.x, q.mu);
     return false;
   }
+  // first j with row[j] * (1/sigma_s) >= U, clamped to G - 1 (the row is
+  // cumulative, so this is a count over the first G - 1 entries)
   const Real *row = &T.cumrow[(m * G + q.g) * G];
   const Real is = T.invs[xs];
-  int gn = G - 1;
-  for (int j = 0; j < G; j++) {
-    if (!(row[j] * is < u_scat)) {
-      gn = j;
-      break;
-    }
-  }
+  int gn = 0;
+  for (int j = 0; j < G - 1; j++) gn += row[j] * is < u_scat ? 1 : 0;
   trace_event<F>(C, q, NRAPS_EV_SCATTER, q.cell, q.g, gn, q.x, q.mu);
   q.g = gn;
   q.mu = mu_new;

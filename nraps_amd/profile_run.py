@@ -43,6 +43,7 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "fp64"], default=None)
     ap.add_argument("--method", choices=["event", "plain"], default=None)
     ap.add_argument("--copy-mib", type=int, default=0)
+    ap.add_argument("--chunk", type=int, default=0, help="histories per wave claim (0: automatic)")
     a = ap.parse_args()
     p, o = nr.generate(a.config, a.index)
     fission = o.source == _abi.SOURCE_FISSION_BANK
@@ -51,6 +52,8 @@ def main():
     o.use_graph = 0
     if a.histories:
         o.histories = a.histories
+    if a.chunk:
+        o.chunk = a.chunk
     if a.precision:
         o.precision = _abi.FP32 if a.precision == "fp32" else _abi.FP64
     if a.method:
@@ -60,6 +63,7 @@ def main():
     r = nr.solve(p, o)
     print(f"config {a.config}: {o.histories} x {o.generations} histories, "
           f"transport {r.ms_transport / max(r.transport_launches, 1):.3f} ms/launch, "
+          f"device {r.ms_device / o.generations:.3f} ms/generation, "
           f"k {r.k_mean:.6f}", flush=True)
     if a.copy_mib:
         copy_probe(a.copy_mib)
