@@ -1,6 +1,6 @@
 """Build libnraps.so in-tree: HIP kernels for gfx950 plus the C++ host code.
 
-    python -m nraps_amd.build          # compile and link nraps_amd/libnraps.so
+    python -m nraps_amd.build          # nraps_amd/libnraps.so and the nraps_run driver
 
 Every source is compiled with -ffp-contract=off (the FP64 histories must
 match a CPU replay bit for bit) and packed FP32 instructions disabled.
@@ -15,6 +15,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnraps.so")
+CLI = os.path.join(HERE, "nraps_run")
 ARCH = os.environ.get("NRAPS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["host.cpp", "kernels.hip", "runtime.hip", "server.cpp"]
@@ -63,7 +64,23 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")
     os.replace(tmp, LIB)
+    _build_cli()
     return LIB
+
+
+def _build_cli():
+    """nraps_run: the deck -> solve -> CSV command-line driver (host code
+    only, linked against libnraps.so found next to it)."""
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if not cxx:
+        raise RuntimeError("no host C++ compiler for the command-line driver")
+    tmp = CLI + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "cli.cpp"), "-o", tmp,
+           "-L", HERE, "-lnraps", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stderr}")
+    os.replace(tmp, CLI)
 
 
 if __name__ == "__main__":

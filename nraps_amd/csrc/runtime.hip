@@ -426,9 +426,13 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     grid_ = n_cu_ * blocks_per_cu_;
     const int64_t waves = (int64_t)grid_ * (variant_.block_size() / 64);
     int64_t c = total / (waves * 8);
-    // claims above a few hundred histories leave long tails at the end of a
-    // launch (measured: 1024 is 11% slower than 256 on a 10^7 fission run)
-    c = std::max<int64_t>(64, std::min<int64_t>(256, c));
+    // about 8 claims per wave and launch; the cap bounds the tail a last
+    // claim leaves at the end of the launch (forced 1024-history claims are
+    // 11% slower than automatic ones on a 10^7-history fission run) while
+    // large serving mixes (10^8 histories per launch) still get few atomics
+    // per wave (a cap of 256 there costs 16% of throughput)
+    const int claim_max = std::max(64, env_int("NRAPS_CLAIM_MAX", 1024));
+    c = std::max<int64_t>(64, std::min<int64_t>(claim_max, c));
     claim_ = (int)((c + 63) / 64 * 64);
     if (n_ == 1 && ps[0]->o.chunk > 0) claim_ = (int)std::min<int64_t>(ps[0]->o.chunk, 1 << 20);
   }

@@ -7,6 +7,7 @@ without graph replay.  Fluxes are atomically accumulated (FP32 per-block
 partials in the precision form) and are compared to a stated tolerance.
 """
 import math
+import os
 import time
 
 import numpy as np
@@ -242,3 +243,25 @@ def test_fp32_tally_reduction_error(gpu, monkeypatch):
     assert np.all(err < 1e-5)
     fa, fb = a.flux_mean.reshape(G, -1), b.flux_mean.reshape(G, -1)
     assert np.all(np.abs(fa - fb).max(axis=1) / fb.max(axis=1) < 1e-5)
+
+
+def test_command_line_driver_runs_a_reference_deck(gpu, tmp_path):
+    """nraps_run on the shipped TestCaseA deck: the same k as the library
+    call with the same options, and the three reference CSV files."""
+    import subprocess
+    cli = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "nraps_amd",
+                       "nraps_run")
+    deck = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference",
+                        "TestCaseA.txt")
+    r = subprocess.run([cli, deck, str(tmp_path), "--histories", "20000", "--generations", "8",
+                        "--skip", "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for f in ("vars.csv", "interface.csv", "k_eff.csv"):
+        assert (tmp_path / f).exists()
+    p, _ = nr.load_deck(deck)
+    rb = nr.solve(p, nr.run_opts(histories=20000, generations=8, skip=2, precision=_abi.FP32,
+                                 method=_abi.EVENT_WOODCOCK))
+    k_row = (tmp_path / "k_eff.csv").read_text().splitlines()[0].split(",")
+    assert len(k_row) == 8
+    assert abs(float(k_row[-1]) - rb.k[-1]) <= 1e-12 * abs(rb.k[-1])
+    assert f"{rb.k_mean:.8f}" in r.stdout

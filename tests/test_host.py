@@ -222,6 +222,25 @@ def test_no_gpu_means_an_error_not_a_fallback():
     assert "device" in nr.last_error()
 
 
+def test_command_line_driver_usage_and_no_gpu_error():
+    """nraps_run (deck -> solve -> CSV) is built beside the library; bad
+    arguments print the usage (status 2) and without a device the solve
+    fails loudly (status 1), with no fallback."""
+    import subprocess
+    cli = os.path.join(ROOT, "nraps_amd", "nraps_run")
+    assert os.access(cli, os.X_OK)
+    r = subprocess.run([cli], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+    r = subprocess.run([cli, os.path.join(REFDATA, "TestCaseA.txt"), "--bogus"], capture_output=True,
+                       text=True)
+    assert r.returncode == 2
+    from conftest import has_gpu
+    if not has_gpu():
+        r = subprocess.run([cli, os.path.join(REFDATA, "TestCaseA.txt"), "/tmp", "--histories",
+                            "100"], capture_output=True, text=True)
+        assert r.returncode == 1 and "no HIP device" in r.stderr
+
+
 def test_serving_arrivals_use_pcg32():
     """The serving benchmark's arrival generator is PCG32 (pcg-c-basic KAT
     srandom(42, 54)) and gives Poisson arrivals at the requested rate."""
