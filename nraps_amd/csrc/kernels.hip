@@ -695,9 +695,23 @@ struct WaveStats {
 };
 
 // Minimum resident waves per SIMD requested from the register allocator.
+// The FP32 production kernel in 256-thread blocks runs 5 waves per SIMD (96
+// VGPRs; the spills land in the refill and boundary paths): measured 5.6%
+// faster than 4 waves with 127 VGPRs and no spills, and 6 waves is slower.
+// A 1024-thread block keeps 4 waves per SIMD whatever the register count,
+// and the FP64, trace and fission variants carry more live state, so those
+// keep the allocator's own choice (at least 3 waves).
 #ifndef NRAPS_WC_WAVES_PER_EU
 #define NRAPS_WC_WAVES_PER_EU 3
 #endif
+#ifndef NRAPS_WC_WAVES_PER_EU_FAST
+#define NRAPS_WC_WAVES_PER_EU_FAST 5
+#endif
+template <typename Real, bool TRACE, bool FISSION, int BS>
+constexpr int wc_waves_per_eu() {
+  return (sizeof(Real) == 4 && !TRACE && !FISSION && BS == 256) ? NRAPS_WC_WAVES_PER_EU_FAST
+                                                                 : NRAPS_WC_WAVES_PER_EU;
+}
 
 // Persistent event kernel.  A block visits the problems of the batch in
 // cyclic order starting at its own index; on a problem with histories left
@@ -712,7 +726,8 @@ struct WaveStats {
 // block per CU uses the whole LDS.
 // Dynamic LDS: [hot tables][tally direct|diff].
 template <typename Real, bool LDS_TALLY, bool TRACE, bool FISSION, int BS>
-__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(NRAPS_WC_WAVES_PER_EU)))
+__global__ __launch_bounds__(BS)
+__attribute__((amdgpu_waves_per_eu(wc_waves_per_eu<Real, TRACE, FISSION, BS>())))
 void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim, int refill_min) {
   using F = Feat<TRACE, FISSION>;
   extern __shared__ __align__(16) uint8_t smem[];

@@ -44,8 +44,11 @@ def main():
     ap.add_argument("--method", choices=["event", "plain"], default=None)
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0, help="histories per wave claim (0: automatic)")
+    ap.add_argument("--bank", action="store_true", help="fission-bank source (skip >= 2)")
     a = ap.parse_args()
     p, o = nr.generate(a.config, a.index)
+    if a.bank:
+        o.source = _abi.SOURCE_FISSION_BANK
     fission = o.source == _abi.SOURCE_FISSION_BANK
     o.generations = max(a.generations, 3 if fission else 2)
     o.skip = 2 if fission else 1
