@@ -694,23 +694,21 @@ struct WaveStats {
   }
 };
 
-// Minimum resident waves per SIMD requested from the register allocator.
-// The FP32 production kernel in 256-thread blocks runs 5 waves per SIMD (96
-// VGPRs; the spills land in the refill and boundary paths): measured 5.6%
-// faster than 4 waves with 127 VGPRs and no spills, and 6 waves is slower.
-// A 1024-thread block keeps 4 waves per SIMD whatever the register count,
-// and the FP64, trace and fission variants carry more live state, so those
-// keep the allocator's own choice (at least 3 waves).
+// Minimum resident waves per SIMD requested from the register allocator
+// (measured on one MI355X):
+//   FP32, 256-thread blocks, no fission: 5 waves (96 VGPRs, spills only in
+//     the refill and boundary paths), 5.6% faster than 4 waves with 127
+//     VGPRs and no spills; 6 waves is slower;
+//   FP32, 256-thread blocks, fission sites: 4 waves (128 VGPRs), 4.5% faster
+//     than the allocator's 156 VGPRs at 3 waves;
+//   1024-thread blocks run 4 waves per SIMD whatever the count; the FP64
+//     and trace variants keep the allocator's choice (at least 3 waves).
 #ifndef NRAPS_WC_WAVES_PER_EU
 #define NRAPS_WC_WAVES_PER_EU 3
 #endif
-#ifndef NRAPS_WC_WAVES_PER_EU_FAST
-#define NRAPS_WC_WAVES_PER_EU_FAST 5
-#endif
 template <typename Real, bool TRACE, bool FISSION, int BS>
 constexpr int wc_waves_per_eu() {
-  return (sizeof(Real) == 4 && !TRACE && !FISSION && BS == 256) ? NRAPS_WC_WAVES_PER_EU_FAST
-                                                                 : NRAPS_WC_WAVES_PER_EU;
+  return (sizeof(Real) == 4 && !TRACE && BS == 256) ? (FISSION ? 4 : 5) : NRAPS_WC_WAVES_PER_EU;
 }
 
 // Persistent event kernel.  A block visits the problems of the batch in
