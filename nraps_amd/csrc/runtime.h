@@ -93,10 +93,14 @@ class ProblemDev {
   DevProblem dev{};
   DevFinal fin{};
   CycleState *d_cs = nullptr;
-  double *d_kref = nullptr, *d_flux_acc = nullptr, *d_fs_acc = nullptr, *d_fsum = nullptr,
-         *d_fsq = nullptr, *d_tally_last = nullptr;
-  int64_t *d_sites = nullptr;
-  unsigned long long *d_stats = nullptr;
+  // byte offsets inside the block of small per-problem arrays: [tab .. mat)
+  // is uploaded from `stage`, [up_end, end) holds the zeroed accumulators
+  struct Blk {
+    size_t tab, cs, dx, width, nut, sigf, mat, up_end;
+    size_t k, kref, sites, flux, fs, fsum, fsq, tl, stats, end;
+  } blk{};
+  uint8_t *d_base = nullptr;
+  std::vector<uint8_t> stage;
   DevArena arena;
 };
 

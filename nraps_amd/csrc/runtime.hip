@@ -123,11 +123,59 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
   const Prepared &P = prep;
   const int64_t N = o.histories;
   const int rs = fp32 ? 4 : 8;
+  const int nc = P.nc, GM = P.G * P.M, Gnc = P.G * P.nc, gens = o.generations;
+
+  // Every small per-problem array lives in one block: the part the host
+  // fills (tables, initial cycle state, finalize inputs) goes up in one
+  // copy from a staging buffer, the accumulators after it are zeroed by one
+  // fill and come back in one copy.
+  size_t at = 0;
+  auto put = [&](size_t bytes) {
+    size_t off = at;
+    at = (at + bytes + 255) & ~(size_t)255;
+    return off;
+  };
+  Blk &B = blk;
+  B.tab = put(blob.size());
+  B.cs = cs_slot ? 0 : put(sizeof(CycleState));
+  B.dx = put((size_t)nc * 8);
+  B.width = put((size_t)nc * 8);
+  B.nut = put((size_t)GM * 8);
+  B.sigf = put((size_t)GM * 8);
+  B.mat = put((size_t)nc * 4);
+  B.up_end = at;
+  B.k = put((size_t)gens * 8);
+  B.kref = put((size_t)gens * 8);
+  B.sites = put((size_t)gens * 8);
+  B.flux = put((size_t)Gnc * 8);
+  B.fs = put((size_t)nc * 8);
+  B.fsum = put((size_t)Gnc * 8);
+  B.fsq = put((size_t)Gnc * 8);
+  B.tl = put((size_t)Gnc * 8);
+  B.stats = put(sizeof(unsigned long long) * kNumStats);
+  B.end = at;
+  uint8_t *base = arena.alloc<uint8_t>(B.end);
+  d_base = base;
+
+  std::memset(&cs0, 0, sizeof cs0);
+  cs0.k_prev = o.k_initial;
+  cs0.active = 1;
+  stage.assign(B.up_end, 0);
+  std::memcpy(&stage[B.tab], blob.data(), blob.size());
+  if (!cs_slot) std::memcpy(&stage[B.cs], &cs0, sizeof cs0);
+  std::memcpy(&stage[B.dx], pb->delta_x, (size_t)nc * 8);
+  std::memcpy(&stage[B.width], width.data(), (size_t)nc * 8);
+  std::memcpy(&stage[B.nut], pb->nut, (size_t)GM * 8);
+  std::memcpy(&stage[B.sigf], pb->sigf, (size_t)GM * 8);
+  std::memcpy(&stage[B.mat], pb->matid, (size_t)nc * 4);
+  h2d(base, stage.data(), B.up_end);
+  zero(base + B.up_end, B.end - B.up_end);
+  d_cs = cs_slot ? cs_slot : reinterpret_cast<CycleState *>(base + B.cs);
+  if (cs_slot) h2d(cs_slot, &cs0, sizeof cs0);
+
   DevProblem &d = dev;
   std::memset(&d, 0, sizeof d);
-  uint8_t *tab = arena.alloc<uint8_t>(blob.size());
-  h2d(tab, blob.data(), blob.size());
-  d.tab = tab;
+  d.tab = base + B.tab;
   d.lay = lay;
   d.nc = P.nc;
   d.G = P.G;
@@ -149,13 +197,9 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
   d.XR = P.XR;
   d.maj = P.maj;
   d.inv_maj = 1.0 / P.maj;
-  std::memset(&cs0, 0, sizeof cs0);
-  cs0.k_prev = o.k_initial;
-  cs0.active = 1;
-  d_cs = cs_slot ? cs_slot : arena.alloc<CycleState>(1);
-  h2d(d_cs, &cs0, sizeof cs0);
   d.cs = d_cs;
-  d.k = arena.alloc<double>(o.generations);
+  d.k = reinterpret_cast<double *>(base + B.k);
+  d.stats = reinterpret_cast<unsigned long long *>(base + B.stats);
   d.kacc = arena.alloc<double>(N);
   d.tally_glob = arena.alloc<double>(d.tally_bins);
   if (fission) {
@@ -185,45 +229,26 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
     d.trace_ev = o.trace_events;
     d.trace_cycle = o.trace_cycle;
   }
-  d_stats = arena.alloc<unsigned long long>(kNumStats);
-  zero(d_stats, sizeof(unsigned long long) * kNumStats);
-  d.stats = d_stats;
 
   DevFinal &f = fin;
   std::memset(&f, 0, sizeof f);
-  const int nc = P.nc, GM = P.G * P.M, Gnc = P.G * P.nc;
-  auto up_d = [&](const double *src, size_t cnt) {
-    double *p = arena.alloc<double>(cnt);
-    h2d(p, src, cnt * 8);
-    return p;
-  };
-  f.dx = up_d(pb->delta_x, nc);
-  f.width = up_d(width.data(), nc);
-  f.nut = up_d(pb->nut, GM);
-  f.sigf = up_d(pb->sigf, GM);
-  int32_t *mat = arena.alloc<int32_t>(nc);
-  h2d(mat, pb->matid, nc * 4);
-  f.mat = mat;
+  f.dx = reinterpret_cast<const double *>(base + B.dx);
+  f.width = reinterpret_cast<const double *>(base + B.width);
+  f.nut = reinterpret_cast<const double *>(base + B.nut);
+  f.sigf = reinterpret_cast<const double *>(base + B.sigf);
+  f.mat = reinterpret_cast<const int32_t *>(base + B.mat);
   f.nut_conv = P.nut_conv;
   f.length = P.XR;
   f.generations = o.generations;
   f.skip = o.skip;
   f.tally_cycle = d.tally_glob;
-  f.tally_last = d_tally_last = arena.alloc<double>(Gnc);
-  f.flux_acc = d_flux_acc = arena.alloc<double>(Gnc);
-  f.fs_acc = d_fs_acc = arena.alloc<double>(nc);
-  f.fsum = d_fsum = arena.alloc<double>(Gnc);
-  f.fsq = d_fsq = arena.alloc<double>(Gnc);
-  f.k_ref = d_kref = arena.alloc<double>(o.generations);
-  f.sites_hist = d_sites = arena.alloc<int64_t>(o.generations);
-  zero(d_flux_acc, Gnc * 8);
-  zero(d_fs_acc, nc * 8);
-  zero(d_fsum, Gnc * 8);
-  zero(d_fsq, Gnc * 8);
-  zero(d_tally_last, Gnc * 8);
-  zero(d.k, o.generations * 8);
-  zero(d_kref, o.generations * 8);
-  zero(d_sites, o.generations * 8);
+  f.tally_last = reinterpret_cast<double *>(base + B.tl);
+  f.flux_acc = reinterpret_cast<double *>(base + B.flux);
+  f.fs_acc = reinterpret_cast<double *>(base + B.fs);
+  f.fsum = reinterpret_cast<double *>(base + B.fsum);
+  f.fsq = reinterpret_cast<double *>(base + B.fsq);
+  f.k_ref = reinterpret_cast<double *>(base + B.kref);
+  f.sites_hist = reinterpret_cast<int64_t *>(base + B.sites);
   f.has_diff = woodcock ? 1 : 0;
 }
 
@@ -244,19 +269,9 @@ int ProblemDev::download(nraps_result *r, double ms_total, double ms_device, dou
                    : hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
   };
   CycleState cs;
-  std::vector<double> k(gens), kref(gens), flux(Gnc), fs(nc), fsum(Gnc), fsq(Gnc), tl(Gnc);
-  std::vector<int64_t> sites(gens);
-  unsigned long long st[kNumStats];
+  std::vector<uint8_t> acc(blk.end - blk.up_end);
   d2h(&cs, d_cs, sizeof cs);
-  d2h(k.data(), dev.k, gens * 8);
-  d2h(kref.data(), d_kref, gens * 8);
-  d2h(sites.data(), d_sites, gens * 8);
-  d2h(flux.data(), d_flux_acc, Gnc * 8);
-  d2h(fs.data(), d_fs_acc, nc * 8);
-  d2h(fsum.data(), d_fsum, Gnc * 8);
-  d2h(fsq.data(), d_fsq, Gnc * 8);
-  d2h(tl.data(), d_tally_last, Gnc * 8);
-  d2h(st, d_stats, sizeof st);
+  d2h(acc.data(), d_base + blk.up_end, acc.size());
   if (r && trace_n > 0) {
     d2h(r->trace, dev.trace, sizeof(nraps_trace_event) * trace_n * o.trace_events);
     d2h(r->trace_len, dev.trace_len, 4 * trace_n);
@@ -264,6 +279,21 @@ int ProblemDev::download(nraps_result *r, double ms_total, double ms_device, dou
     d2h(r->kacc, dev.trace_kacc, 8 * trace_n);
   }
   if (stream) HIP_TRY(hipStreamSynchronize(stream));
+  auto take = [&](size_t off, size_t n, void *dst) {
+    std::memcpy(dst, &acc[off - blk.up_end], n);
+  };
+  std::vector<double> k(gens), kref(gens), flux(Gnc), fs(nc), fsum(Gnc), fsq(Gnc), tl(Gnc);
+  std::vector<int64_t> sites(gens);
+  unsigned long long st[kNumStats];
+  take(blk.k, (size_t)gens * 8, k.data());
+  take(blk.kref, (size_t)gens * 8, kref.data());
+  take(blk.sites, (size_t)gens * 8, sites.data());
+  take(blk.flux, (size_t)Gnc * 8, flux.data());
+  take(blk.fs, (size_t)nc * 8, fs.data());
+  take(blk.fsum, (size_t)Gnc * 8, fsum.data());
+  take(blk.fsq, (size_t)Gnc * 8, fsq.data());
+  take(blk.tl, (size_t)Gnc * 8, tl.data());
+  take(blk.stats, sizeof st, st);
   // generations actually run (the server may stop a problem early)
   const int ran = std::min(gens, (int)cs.cycle);
   const int skip = o.skip;
