@@ -71,12 +71,14 @@ def main():
     for i in range(args.warmup):
         nr.solve(prob, o)
         log(f"warmup {i + 1}/{args.warmup} done")
-    tr_ms, tr_launch, ks = 0.0, 0, []
+    tr_ms, tr_launch, dev_ms, tot_ms, ks = 0.0, 0, 0.0, 0.0, []
     t0 = time.perf_counter()
     for i in range(args.steps):
         r = nr.solve(prob, o)
         tr_ms += r.ms_transport
         tr_launch += r.transport_launches
+        dev_ms += r.ms_device
+        tot_ms += r.ms_total
         ks.append((r.k_mean, r.k_sigma))
         log(f"step {i + 1}/{args.steps}: k = {r.k_mean:.6f} +- {r.k_sigma:.6f}")
     t1 = time.perf_counter()
@@ -95,6 +97,10 @@ def main():
                                "Woodcock tracking, FP64 k", "histories_per_generation": N,
                    "generations": gens, "cells": prob.n_cells, "groups": prob.n_groups},
         "k_fp32": k32, "k_fp32_sigma": s32,
+        # the solve's own clocks: generation loop on the device (HIP events)
+        # and upload-to-download inside the library
+        "ms_device_per_step": dev_ms / args.steps, "ms_solve_per_step": tot_ms / args.steps,
+        "device_histories_per_s": args.steps * N * gens / (dev_ms * 1e-3),
     }
     # roofline of the dominant kernel (the transport launch)
     b_hist = B_STATE_FP32 * (1 + 2 * fbar)
