@@ -859,22 +859,31 @@ __device__ inline int64_t tree_len(int64_t n, int level) {
 }
 
 __global__ __launch_bounds__(kTreeWidth) void k_ktree(const DevProblem *probs, int level) {
+  static_assert(kTreeWidth == 256, "the tree below is written for 256-wide groups");
   const DevProblem &P = probs[blockIdx.y];
   if (!P.cs->active) return;
-  __shared__ double s[kTreeWidth];
+  __shared__ double s[kTreeWidth / 2];
+  const int t = threadIdx.x;
   const int64_t n_in = tree_len(P.N, level);
   const int64_t n_out = (n_in + kTreeWidth - 1) / kTreeWidth;
   const double *in = level == 0 ? P.kacc : ((level - 1) % 2 == 0 ? P.tree_a : P.tree_b);
   double *out = level % 2 == 0 ? P.tree_a : P.tree_b;
   for (int64_t gi = blockIdx.x; gi < n_out; gi += gridDim.x) {
-    int64_t j = gi * kTreeWidth + threadIdx.x;
-    s[threadIdx.x] = j < n_in ? in[j] : 0.0;
+    const int64_t j = gi * kTreeWidth + t;
+    double v = j < n_in ? in[j] : 0.0;
+    // strides 128 and 64 through LDS: element i takes element i + stride
+    if (t >= 128) s[t - 128] = v;
     __syncthreads();
-    for (int stride = kTreeWidth / 2; stride >= 1; stride >>= 1) {
-      if ((int)threadIdx.x < stride) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + stride];
-      __syncthreads();
+    if (t < 128) v = v + s[t];
+    __syncthreads();
+    if (t >= 64 && t < 128) s[t - 64] = v;
+    __syncthreads();
+    if (t < 64) {
+      v = v + s[t];
+      // strides 32 .. 1 inside the first wave: lane i adds lane i + stride
+      for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_down(v, off);
+      if (t == 0) out[gi] = v;
     }
-    if (threadIdx.x == 0) out[gi] = s[0];
     __syncthreads();
   }
 }
