@@ -28,9 +28,9 @@ HBM_PEAK_GBS = 8000.0
 B_STATE_FP32, B_STATE_FP64 = 24, 32
 # HBM bytes per event-kernel launch on this workload, from two rocprofv3 counter
 # passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors calibrated on a
-# 1 GiB device copy): 90.5 KiB x 2 read + 94743 KiB written, per launch of
-# 10^7 histories (profiles/k_woodcock_c1_hbm_traffic.md).
-TRAFFIC_BYTES_PER_LAUNCH = (90.5 * 2 + 94743.0) * 1024
+# 1 GiB device copy): 15514.7 KiB x 2 read + 159322.7 KiB written, per launch
+# of 10^7 histories (profiles/k_woodcock_c1_hbm_traffic.md).
+TRAFFIC_BYTES_PER_LAUNCH = (15514.7 * 2 + 159322.7) * 1024
 
 
 def main():

@@ -54,6 +54,13 @@ struct Num<float> {
 };
 
 __device__ inline int lane_id() { return (int)__lane_id(); }
+// Lane 0's value, read into scalar registers (the caller knows it is the
+// same in every lane): wave-uniform counters then take no vector registers.
+__device__ inline uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
 __device__ inline int rank_below(uint64_t mask) {
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -325,14 +332,14 @@ struct SiteCursor {
         close(C);
         unsigned long long b = 0;
         if (lane_id() == 0) b = atomicAdd(C.site_count, (unsigned long long)kSiteChunk);
-        next = __shfl(b, 0);
+        next = uniform_u64(b);
         end = next + kSiteChunk;
       }
       if (want) {
         put_site(C, next + (unsigned long long)rank_below(m), q);
         rem--;
       }
-      next += cnt;
+      next = uniform_u64(next + cnt);
     }
   }
 };
@@ -778,7 +785,7 @@ void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim
           if (r_cur >= r_end) {
             unsigned long long b = 0;
             if (lane_id() == 0) b = atomicAdd(queue, (unsigned long long)claim);
-            b = __shfl(b, 0);
+            b = uniform_u64(b);
             if ((int64_t)b >= C.N) {
               queue_done = true;
               break;
