@@ -96,6 +96,7 @@ struct DevFinal {
   double *flux_acc, *fs_acc, *fsum, *fsq; // accumulators
   double *k_ref;        // [generations]
   int64_t *sites_hist;  // [generations]
+  double *kpart;        // [G] per-group share of the reference k (scratch)
   int32_t has_diff;     // tally_cycle carries a difference array
 };
 

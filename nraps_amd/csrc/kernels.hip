@@ -20,7 +20,8 @@
 //                      fission bank: sites placed in (history, sequence) order
 //                      by a prefix sum of per-history counts, then systematic
 //                      resampling to N sources in FP64
-//   k_finalize         k, reference normalisation and accumulation
+//   k_finalize_rows, k_finalize_close
+//                      k, reference normalisation and accumulation
 //                      (src/mc_code.rs:340-363), variance sums, next-cycle state
 //
 // Compiled with -ffp-contract=off: the FP64 histories must reproduce the CPU
@@ -64,6 +65,31 @@ __device__ inline uint64_t uniform_u64(uint64_t v) {
 __device__ inline int rank_below(uint64_t mask) {
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Exclusive prefix sum over a 256-thread block (every thread calls it):
+// inclusive scans inside each wave by shuffles, then the wave totals.
+// *total receives the block's sum.  The order of the additions differs from
+// a serial scan, which matters for floating point only.
+template <typename T>
+__device__ inline T block_exclusive_scan(T v, T *total) {
+  __shared__ T wsum[kBlock / 64];
+  const int lane = lane_id(), w = (int)(threadIdx.x >> 6);
+  T inc = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const T u = __shfl_up(inc, off);
+    if (lane >= off) inc = inc + u;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  T base = T(0), tot = T(0);
+  for (int i = 0; i < kBlock / 64; i++) {
+    if (i < w) base = base + wsum[i];
+    tot = tot + wsum[i];
+  }
+  __syncthreads();  // wsum may be reused by the next call
+  *total = tot;
+  return base + (inc - v);
 }
 
 // Views of the packed tables: the hot part (in LDS or global memory) and the
@@ -533,7 +559,7 @@ __device__ inline int locate(const Ctx<Real> &C, Real xe, bool right) {
 // Track-length tally of a straight piece from (xa, ca) to (xb, cb) with
 // weight w = 1/|mu|: the partial end cells are scored directly, the cells
 // in between by two updates of a difference array.  The generation tally
-// is direct + width * prefix_sum(difference) (done in k_finalize).
+// is direct + width * prefix_sum(difference) (done in k_finalize_rows).
 template <typename Real, typename Acc>
 __device__ inline void tally_piece(Acc *direct, Acc *diff, const Real *edges, int base, Real xa,
                                    int ca, Real xb, int cb, Real w) {
@@ -935,15 +961,9 @@ __global__ __launch_bounds__(kBlock) void k_scan_tilesums(const DevProblem *prob
   const int64_t lo = threadIdx.x * per, hi = lo + per < ntiles ? lo + per : ntiles;
   int64_t s = 0;
   for (int64_t t = lo; t < hi; t++) s += ts[t];
-  part[threadIdx.x] = s;
-  __syncthreads();
+  int64_t run;
+  part[threadIdx.x] = block_exclusive_scan<int64_t>(s, &run);
   if (threadIdx.x == 0) {
-    int64_t run = 0;
-    for (int i = 0; i < kBlock; i++) {
-      int64_t v = part[i];
-      part[i] = run;
-      run += v;
-    }
     CycleState *cs = P.cs;
     cs->n_sites = run;
     if (run == 0) cs->error |= 2;
@@ -952,11 +972,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_tilesums(const DevProblem *prob
     if ((int64_t)cs->site_count > P.site_cap || (int64_t)cs->site_count < run) cs->error |= 1;
   }
   __syncthreads();
-  int64_t run = part[threadIdx.x];
+  int64_t r = part[threadIdx.x];
   for (int64_t t = lo; t < hi; t++) {
     int64_t v = ts[t];
-    ts[t] = run;
-    run += v;
+    ts[t] = r;
+    r += v;
   }
 }
 
@@ -966,34 +986,24 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const DevProblem *probs, 
   if (!P.cs->active || !P.cs->make_sites) return;
   const int64_t ntiles = (P.N + kScanTile - 1) / kScanTile;
   constexpr int per = kScanTile / kBlock;
-  __shared__ int64_t part[kBlock];
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const int64_t b0 = t * kScanTile + (int64_t)threadIdx.x * per;
+    int64_t c[per];
     int64_t s = 0;
+#pragma unroll
     for (int i = 0; i < per; i++) {
-      int64_t j = b0 + i;
-      if (j < P.N) s += P.counts[j];
+      const int64_t j = b0 + i;
+      c[i] = j < P.N ? P.counts[j] : 0;
+      s += c[i];
     }
-    part[threadIdx.x] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      int64_t run = tile_sums[tile_base[blockIdx.y] + t];
-      for (int i = 0; i < kBlock; i++) {
-        int64_t v = part[i];
-        part[i] = run;
-        run += v;
-      }
-    }
-    __syncthreads();
-    int64_t run = part[threadIdx.x];
+    int64_t tot;
+    int64_t run = tile_sums[tile_base[blockIdx.y] + t] + block_exclusive_scan<int64_t>(s, &tot);
+#pragma unroll
     for (int i = 0; i < per; i++) {
-      int64_t j = b0 + i;
-      if (j < P.N) {
-        P.offsets[j] = run;
-        run += P.counts[j];
-      }
+      const int64_t j = b0 + i;
+      if (j < P.N) P.offsets[j] = run;
+      run += c[i];
     }
-    __syncthreads();
   }
 }
 
@@ -1039,84 +1049,94 @@ template __global__ void k_resample<double>(const DevProblem *);
 // ---------------------------------------------------------------------------
 // Generation close: k, the reference normalisation/accumulation
 // (src/mc_code.rs:340-363), variance sums, next-generation state.
-// One block per problem.
+// k_finalize_rows: one block per (group, problem) rebuilds the cell tally of
+// its group row from the difference array and updates that row's
+// accumulators; the per-cell fission-source terms go to the consumed
+// difference row and the row's share of the reference k to kpart[g].
+// k_finalize_close: one block per problem sums those over groups in group
+// order and advances the generation state.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_finalize(const DevProblem *probs, const DevFinal *fins,
-                                                     int levels) {
-  const DevProblem &P = probs[blockIdx.x];
-  const DevFinal &F = fins[blockIdx.x];
-  CycleState *csp = P.cs;
-  if (!csp->active) return;
+__global__ __launch_bounds__(kBlock) void k_finalize_rows(const DevProblem *probs,
+                                                          const DevFinal *fins) {
+  const DevProblem &P = probs[blockIdx.y];
+  const DevFinal &F = fins[blockIdx.y];
+  const int g = (int)blockIdx.x;
+  if (!P.cs->active || g >= P.G) return;
   const int nc = P.nc, G = P.G, M = P.M;
-  const int cyc = csp->cycle;
-  const double kprev = csp->k_prev;
+  const int cyc = P.cs->cycle;
+  const double kprev = P.cs->k_prev;
   const double N = (double)P.N;
-  const double *tot = (levels - 1) % 2 == 0 ? P.tree_a : P.tree_b;
-  const double knew = tot[0] / N;
-  double *T = F.tally_cycle;  // direct part becomes the cell tally
-  __shared__ double part[kBlock];
-  // cell tally = direct + width * inclusive_prefix(diff), per group
+  double *row = F.tally_cycle + g * nc;             // direct part -> cell tally
+  double *drow = F.tally_cycle + G * nc + g * nc;   // difference row -> scratch
   if (F.has_diff) {
+    // cell tally = direct + width * inclusive_prefix(diff)
     const int per = (nc + kBlock - 1) / kBlock;
     const int lo = threadIdx.x * per, hi = lo + per < nc ? lo + per : nc;
-    for (int g = 0; g < G; g++) {
-      const double *d = T + G * nc + g * nc;
-      double s = 0.0;
-      for (int c = lo; c < hi; c++) s += d[c];
-      part[threadIdx.x] = s;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        double run = 0.0;
-        for (int i = 0; i < kBlock; i++) {
-          double v = part[i];
-          part[i] = run;
-          run += v;
-        }
-      }
-      __syncthreads();
-      double run = part[threadIdx.x];
-      for (int c = lo; c < hi; c++) {
-        run += d[c];
-        T[g * nc + c] += F.width[c] * run;
-      }
-      __syncthreads();
+    double s = 0.0;
+    for (int c = lo; c < hi; c++) s += drow[c];
+    double total;
+    double run = block_exclusive_scan<double>(s, &total);
+    for (int c = lo; c < hi; c++) {
+      run += drow[c];
+      row[c] += F.width[c] * run;
     }
+    __syncthreads();  // the difference row is reused below
   }
   const bool active_gen = cyc >= F.skip;
   const double fund = 1.0 / (double)(F.generations - (F.skip - 1));
   const double conversion =
       (3565e6 * kprev * 36.2) / (200e6 * 1.602176634e-19 * F.nut_conv * F.length);
+  const bool last = cyc == F.generations - 1 && F.tally_last;
   double kref = 0.0;
   for (int c = threadIdx.x; c < nc; c += blockDim.x) {
     const double dxc = F.dx[c];
-    const int mc = F.mat[c];
-    double fs_c = 0.0;
-    for (int g = 0; g < G; g++) {
-      const int xi = mc + M * g;
-      const double t = T[g * nc + c];
-      double flux = t / (kprev * N * dxc);
-      double fsrc = F.nut[xi] * F.sigf[xi] * flux;
-      kref += kprev * dxc * fsrc;
-      if (active_gen) {
-        F.flux_acc[g * nc + c] += flux * conversion * fund;
-        fs_c += fsrc * fund;
-        double fv = t / (N * dxc);
-        F.fsum[g * nc + c] += fv;
-        F.fsq[g * nc + c] += fv * fv;
-      }
-      if (cyc == F.generations - 1 && F.tally_last) F.tally_last[g * nc + c] = t;
+    const int xi = F.mat[c] + M * g;
+    const double t = row[c];
+    const double flux = t / (kprev * N * dxc);
+    const double fsrc = F.nut[xi] * F.sigf[xi] * flux;
+    kref += kprev * dxc * fsrc;
+    if (active_gen) {
+      F.flux_acc[g * nc + c] += flux * conversion * fund;
+      drow[c] = fsrc * fund;
+      const double fv = t / (N * dxc);
+      F.fsum[g * nc + c] += fv;
+      F.fsq[g * nc + c] += fv * fv;
     }
-    if (active_gen) F.fs_acc[c] += fs_c;
+    if (last) F.tally_last[g * nc + c] = t;
   }
+  __shared__ double part[kBlock];
   part[threadIdx.x] = kref;
   __syncthreads();
   for (int stride = kBlock / 2; stride >= 1; stride >>= 1) {
     if ((int)threadIdx.x < stride) part[threadIdx.x] += part[threadIdx.x + stride];
     __syncthreads();
   }
+  if (threadIdx.x == 0) F.kpart[g] = part[0];
+}
+
+__global__ __launch_bounds__(kBlock) void k_finalize_close(const DevProblem *probs,
+                                                           const DevFinal *fins, int levels) {
+  const DevProblem &P = probs[blockIdx.x];
+  const DevFinal &F = fins[blockIdx.x];
+  CycleState *csp = P.cs;
+  if (!csp->active) return;
+  const int nc = P.nc, G = P.G;
+  const int cyc = csp->cycle;
+  const double *tot = (levels - 1) % 2 == 0 ? P.tree_a : P.tree_b;
+  const double knew = tot[0] / (double)P.N;
+  if (cyc >= F.skip) {
+    const double *fsr = F.tally_cycle + G * nc;  // fission-source terms, [G][nc]
+    for (int c = threadIdx.x; c < nc; c += blockDim.x) {
+      double fs_c = 0.0;
+      for (int g = 0; g < G; g++) fs_c += fsr[g * nc + c];
+      F.fs_acc[c] += fs_c;
+    }
+  }
   if (threadIdx.x == 0) {
+    double kref = 0.0;
+    for (int g = 0; g < G; g++) kref += F.kpart[g];
     P.k[cyc] = knew;
-    F.k_ref[cyc] = part[0];
+    F.k_ref[cyc] = kref;
     F.sites_hist[cyc] = csp->make_sites ? csp->n_sites : 0;
     const int next = cyc + 1;
     csp->k_prev = knew;
@@ -1224,8 +1244,11 @@ void launch_bank(bool fp32, const DevProblem *d, int nprob, int64_t max_n, int64
   }
 }
 
-void launch_finalize(const DevProblem *d, const DevFinal *f, int nprob, int levels, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3((unsigned)nprob), dim3(kBlock), 0, s, d, f, levels);
+void launch_finalize(const DevProblem *d, const DevFinal *f, int nprob, int max_groups, int levels,
+                     hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize_rows, dim3((unsigned)max_groups, (unsigned)nprob), dim3(kBlock), 0, s,
+                     d, f);
+  hipLaunchKernelGGL(k_finalize_close, dim3((unsigned)nprob), dim3(kBlock), 0, s, d, f, levels);
 }
 
 void launch_pcg32_fill(uint64_t seed, uint64_t stream0, int64_t n, int draws, uint32_t *out,

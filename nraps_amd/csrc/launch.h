@@ -26,7 +26,8 @@ hipError_t woodcock_set_lds_limit(const WcVariant &v, size_t lds_bytes);
 void launch_ktree(const DevProblem *d, int nprob, int level, int64_t max_groups, hipStream_t s);
 void launch_bank(bool fp32, const DevProblem *d, int nprob, int64_t max_n, int64_t max_tiles,
                  int64_t *tile_sums, const int64_t *tile_base, hipStream_t s);
-void launch_finalize(const DevProblem *d, const DevFinal *f, int nprob, int levels, hipStream_t s);
+void launch_finalize(const DevProblem *d, const DevFinal *f, int nprob, int max_groups, int levels,
+                     hipStream_t s);
 void launch_pcg32_fill(uint64_t seed, uint64_t stream0, int64_t n, int draws, uint32_t *out,
                        hipStream_t s);
 

@@ -153,6 +153,7 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
   B.fsq = put((size_t)Gnc * 8);
   B.tl = put((size_t)Gnc * 8);
   B.stats = put(sizeof(unsigned long long) * kNumStats);
+  B.kpart = put((size_t)P.G * 8);
   B.end = at;
   uint8_t *base = arena.alloc<uint8_t>(B.end);
   d_base = base;
@@ -249,6 +250,7 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
   f.fsq = reinterpret_cast<double *>(base + B.fsq);
   f.k_ref = reinterpret_cast<double *>(base + B.kref);
   f.sites_hist = reinterpret_cast<int64_t *>(base + B.sites);
+  f.kpart = reinterpret_cast<double *>(base + B.kpart);
   f.has_diff = woodcock ? 1 : 0;
 }
 
@@ -401,6 +403,7 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
   fission_any_ = false;
   max_n_ = 0;
   max_bins_ = 0;
+  max_groups_ = 1;
   levels_ = 1;
   max_tiles_ = 0;
   int tab_max = 0, tally_max = 0;
@@ -413,6 +416,7 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     trace_any |= p.trace_n > 0;
     max_n_ = std::max<int64_t>(max_n_, p.o.histories);
     max_bins_ = std::max(max_bins_, p.dev.tally_bins);
+    max_groups_ = std::max(max_groups_, p.dev.G);
     levels_ = std::max(levels_, ktree_levels(p.o.histories));
     tile_base[i] = tiles_total;
     const int64_t t = ceil_div(p.o.histories, kScanTile);
@@ -530,7 +534,7 @@ void Launcher::enqueue_cycle(hipStream_t s, hipEvent_t tr0, hipEvent_t tr1) {
     launch_ktree(d_probs_, n_, l, groups, s);
   }
   if (fission_any_) launch_bank(fp32_, d_probs_, n_, max_n_, max_tiles_, d_tile_sums_, d_tile_base_, s);
-  launch_finalize(d_probs_, d_fins_, n_, levels_, s);
+  launch_finalize(d_probs_, d_fins_, n_, max_groups_, levels_, s);
   HIP_TRY(hipGetLastError());
 }
 
