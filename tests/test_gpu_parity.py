@@ -105,3 +105,17 @@ def test_fp32_single_histories_follow_fp64(gpu):
             worst = max(worst, float(np.max(np.abs(ta["x"] - tb["x"]))))
     assert same >= 0.99 * o64.histories
     assert worst <= 2e-5
+
+
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+def test_partial_albedo_faces_bit_exact(gpu, oracle, method):
+    """Albedo faces between vacuum and mirror (src/mc_code.rs:55-62: mu -> -b*mu
+    and the remaining displacement scaled by -b), b = 0.5 on the left and
+    0.8 on the right: both FP64 paths against the oracle, history by history."""
+    p, o = small_c0(method=method, precision=_abi.FP64)
+    p.boundl, p.boundr = 0.5, 0.8
+    g = nr.solve(p, o, trace=True)
+    c = oracle.solve(p, o, threads=4, trace=True)
+    assert c.status_code == 0
+    assert_traces_equal(g, c, o.histories)
+    assert np.array_equal(g.k.view(np.uint64), c.k.view(np.uint64))
