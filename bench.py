@@ -29,7 +29,7 @@ B_STATE_FP32, B_STATE_FP64 = 24, 32
 # HBM bytes per event-kernel launch on this workload, from two rocprofv3 counter
 # passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors calibrated on a
 # 1 GiB device copy): 15514.7 KiB x 2 read + 159322.7 KiB written, per launch
-# of 10^7 histories (profiles/k_woodcock_c1_hbm_traffic.md).
+# of 10^7 histories (profiles/k_woodcock_9region_4group_hbm_traffic.md).
 TRAFFIC_BYTES_PER_LAUNCH = (15514.7 * 2 + 159322.7) * 1024
 
 
@@ -59,7 +59,7 @@ def main():
     if st != 0:
         raise SystemExit("no HIP device: " + info)
     with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
-        fbar = json.load(f)["workloads"]["c1"]["mean_flights"]
+        fbar = json.load(f)["workloads"]["slab9_g4_1e7"]["mean_flights"]
 
     prob, o = nr.generate(1)           # 10^7 x (20 + 100), FP32 event Woodcock
     o.use_graph = 0                    # per-generation HIP events around the transport kernel
@@ -92,7 +92,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "9-region 4-group slab (c1): 1e7 particles/generation, "
+        "config": {"workload": "9-region 4-group slab: 1e7 particles/generation, "
                                "20 inactive + 100 active generations, FP32 event-based "
                                "Woodcock tracking, FP64 k", "histories_per_generation": N,
                    "generations": gens, "cells": prob.n_cells, "groups": prob.n_groups},
@@ -150,7 +150,7 @@ def main():
         out["cpu_baseline"] = {"value": cpu_v, "unit": "histories/s", "cores": threads,
                                "kind": "port",
                                "sample": f"FP64 CPU oracle (reference semantics, surface tracking) "
-                                         f"on the c1 problem: {args.cpu_histories} histories x "
+                                         f"on the same 9-region 4-group slab: {args.cpu_histories} histories x "
                                          f"{args.cpu_generations} generations, {wc:.1f} s"}
         out["speedup_vs_cpu"] = value / cpu_v
         log(f"CPU oracle: {cpu_v:.4g} histories/s on {threads} threads")

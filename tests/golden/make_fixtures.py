@@ -1,8 +1,8 @@
 """Write tests/golden/workloads.json: the seeded synthetic workloads, pinned.
 
 For each workload the file holds the generated arrays (so a change of the
-generator shows up as a diff) and, for the configurations used as bench
-lines, the mean number of flights per history F (real collisions + leaks)
+generator shows up as a diff) and, for the 9-region and 201-region slabs,
+the mean number of flights per history F (real collisions + leaks)
 measured by the FP64 CPU oracle at 10^6 histories with reference semantics.
 F enters the algorithmic-bytes model of the roofline:
     B_hist = B_state * (1 + 2 F) [+ 2 B_site for the fission bank].
@@ -53,24 +53,25 @@ def main():
     out = {"seed": 42, "workloads": {}}
     p0, o0 = nr.generate(0)
     f0, n0 = mean_flights(p0)
-    out["workloads"]["c0"] = pin(p0, o0)
+    out["workloads"]["slab9_g4"] = pin(p0, o0)
     _, o1 = nr.generate(1)
-    out["workloads"]["c1"] = {"same_problem_as": "c0", "histories": int(o1.histories),
-                              "generations": int(o1.generations), "skip": int(o1.skip),
-                              "mean_flights": f0, "mean_flights_histories": n0}
+    out["workloads"]["slab9_g4_1e7"] = {"same_problem_as": "slab9_g4",
+                                        "histories": int(o1.histories),
+                                        "generations": int(o1.generations), "skip": int(o1.skip),
+                                        "mean_flights": f0, "mean_flights_histories": n0}
     p2, o2 = nr.generate(2)
     w2 = pin(p2, o2)
     f2, n2 = mean_flights(p2, histories=10**5, source=_abi.SOURCE_FISSION_BANK)
     w2["mean_flights"] = f2
     w2["mean_flights_histories"] = n2
-    out["workloads"]["c2"] = w2
+    out["workloads"]["slab201_g8"] = w2
     for i in range(4):
         p3, o3 = nr.generate(3, i)
-        out["workloads"][f"c3_{i}"] = pin(p3, o3)
+        out["workloads"][f"batch_{i}"] = pin(p3, o3)
     path = os.path.join(HERE, "workloads.json")
     with open(path, "w") as f:
         json.dump(out, f, separators=(",", ":"))
-    print(path, f"F(c0/c1) = {f0:.6f}", f"F(c2) = {f2:.6f}")
+    print(path, f"F(9-region slab) = {f0:.6f}", f"F(201-region slab) = {f2:.6f}")
 
 
 if __name__ == "__main__":

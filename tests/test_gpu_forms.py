@@ -167,9 +167,9 @@ def _seed_stats(run, seeds):
     return ks.mean(), ks.std(ddof=1) / math.sqrt(n), tots.mean(0), tots.std(0, ddof=1) / math.sqrt(n)
 
 
-def test_statistical_agreement_c0(gpu, oracle):
+def test_statistical_agreement_9region_slab(gpu, oracle):
     """FP32 event, FP64 event and the plain FP64 port against the FP64
-    oracle on the c0 problem, eight independent seeds each: k and the group
+    oracle on the 9-region 4-group slab, eight independent seeds each: k and the group
     fluxes within 4 combined standard errors and 0.5%; the FP32 and FP64
     event paths on the same streams agree to 1e-4."""
     p, _ = nr.generate(0)
@@ -190,7 +190,7 @@ def test_statistical_agreement_c0(gpu, oracle):
     assert abs(a[0] / b[0] - 1) < 1e-4 and np.all(np.abs(a[2] / b[2] - 1) < 1e-4)
 
 
-def test_precision_form_c2_fp32_vs_fp64(gpu):
+def test_precision_form_201region_fp32_vs_fp64(gpu):
     """Precision form on the 201-region 8-group fission-bank slab: FP32
     tracking/tallies with FP64 k and bank against the FP64 event path, four
     independent seeds each: k and the eight group fluxes within 4 combined
@@ -213,7 +213,7 @@ def test_precision_form_c2_fp32_vs_fp64(gpu):
     assert np.all(np.abs(b[2] / a[2] - 1) < 5e-3)
 
 
-def test_full_size_c1_invariants(gpu):
+def test_full_size_9region_1e7_invariants(gpu):
     """At the bench size (10^7 histories per generation): every history ends
     exactly once, k from the per-history FP64 sums agrees with k recomputed
     from the FP32 tallies, and the FP64 event path gives the same k within

@@ -105,8 +105,9 @@ def _pinned():
         return json.load(f)["workloads"]
 
 
-@pytest.mark.parametrize("key,config,index", [("c0", 0, 0), ("c2", 2, 0), ("c3_0", 3, 0),
-                                              ("c3_1", 3, 1), ("c3_2", 3, 2), ("c3_3", 3, 3)])
+@pytest.mark.parametrize("key,config,index", [("slab9_g4", 0, 0), ("slab201_g8", 2, 0),
+                                              ("batch_0", 3, 0), ("batch_1", 3, 1),
+                                              ("batch_2", 3, 2), ("batch_3", 3, 3)])
 def test_generators_match_pinned_workloads(key, config, index):
     w = _pinned()[key]
     p, o = nr.generate(config, index)
