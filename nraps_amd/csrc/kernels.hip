@@ -26,6 +26,7 @@
 //
 // Compiled with -ffp-contract=off: the FP64 histories must reproduce the CPU
 // replay bit for bit.
+#include <algorithm>
 #include <type_traits>
 
 #include "device.h"
@@ -1170,6 +1171,15 @@ static inline unsigned clamp_grid(int64_t n, int64_t cap) {
   return (unsigned)(n < cap ? n : cap);
 }
 
+// Blocks per problem for the grid-stride kernels of a batch (blockIdx.y is
+// the problem): enough for the largest problem alone, but with many
+// problems of uneven size a cap of ~64k blocks in the whole launch, since
+// every block of a small problem past its end still costs a dispatch.
+static inline unsigned batch_grid(int64_t n, int64_t cap, int nprob) {
+  const int64_t share = std::max<int64_t>(256, (int64_t)65536 / std::max(nprob, 1));
+  return clamp_grid(n, std::min(cap, share));
+}
+
 void launch_cycle_begin(const DevProblem *d, int nprob, int max_bins, hipStream_t s) {
   dim3 grid(clamp_grid((max_bins + kBlock - 1) / kBlock, 64), (unsigned)nprob);
   hipLaunchKernelGGL(k_cycle_begin, grid, dim3(kBlock), 0, s, d);
@@ -1223,18 +1233,18 @@ int woodcock_blocks_per_cu(const WcVariant &v, size_t lds_bytes) {
 }
 
 void launch_ktree(const DevProblem *d, int nprob, int level, int64_t max_groups, hipStream_t s) {
-  dim3 grid(clamp_grid(max_groups, 65535), (unsigned)nprob);
+  dim3 grid(batch_grid(max_groups, 65535, nprob), (unsigned)nprob);
   hipLaunchKernelGGL(k_ktree, grid, dim3(kTreeWidth), 0, s, d, level);
 }
 
 void launch_bank(bool fp32, const DevProblem *d, int nprob, int64_t max_n, int64_t max_tiles,
                  int64_t *tile_sums, const int64_t *tile_base, hipStream_t s) {
-  dim3 gt(clamp_grid(max_tiles, 65535), (unsigned)nprob);
+  dim3 gt(batch_grid(max_tiles, 65535, nprob), (unsigned)nprob);
   hipLaunchKernelGGL(k_scan_tiles, gt, dim3(kBlock), 0, s, d, tile_sums, tile_base);
   hipLaunchKernelGGL(k_scan_tilesums, dim3((unsigned)nprob), dim3(kBlock), 0, s, d, tile_sums, tile_base);
   hipLaunchKernelGGL(k_scan_apply, gt, dim3(kBlock), 0, s, d, (const int64_t *)tile_sums, tile_base);
-  dim3 gp(clamp_grid((4 * max_n + 65536 + kBlock - 1) / kBlock, 8192), (unsigned)nprob);
-  dim3 gr(clamp_grid((max_n + kBlock - 1) / kBlock, 8192), (unsigned)nprob);
+  dim3 gp(batch_grid((4 * max_n + 65536 + kBlock - 1) / kBlock, 8192, nprob), (unsigned)nprob);
+  dim3 gr(batch_grid((max_n + kBlock - 1) / kBlock, 8192, nprob), (unsigned)nprob);
   if (fp32) {
     hipLaunchKernelGGL(k_place<float>, gp, dim3(kBlock), 0, s, d);
     hipLaunchKernelGGL(k_resample<float>, gr, dim3(kBlock), 0, s, d);
