@@ -75,8 +75,10 @@ class ProblemDev {
   }
   // Copy results into r (one synchronisation of the set-up stream); returns
   // the status of the run.
+  // `via`: stream for the copies instead of the set-up stream (the caller
+  // guarantees the problem's device work has completed).
   int download(nraps_result *r, double ms_total, double ms_device, double ms_transport,
-               int64_t launches);
+               int64_t launches, hipStream_t via = nullptr);
   // Stop the problem from taking part in further launches.
   void deactivate(hipStream_t s);
 

@@ -261,14 +261,15 @@ void ProblemDev::deactivate(hipStream_t s) {
 }
 
 int ProblemDev::download(nraps_result *r, double ms_total, double ms_device, double ms_transport,
-                         int64_t launches) {
+                         int64_t launches, hipStream_t via) {
   const Prepared &P = prep;
   const int nc = P.nc, G = P.G, gens = o.generations, Gnc = G * nc;
-  // every copy is queued on the set-up stream (after the work of the run)
-  // and waited for once
+  // every copy is queued on one stream (by default the set-up stream, after
+  // the work of the run) and waited for once
+  hipStream_t cs_stream = via ? via : stream;
   auto d2h = [&](void *dst, const void *src, size_t n) {
-    HIP_TRY(stream ? hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, stream)
-                   : hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
+    HIP_TRY(cs_stream ? hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, cs_stream)
+                      : hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
   };
   CycleState cs;
   std::vector<uint8_t> acc(blk.end - blk.up_end);
@@ -280,7 +281,7 @@ int ProblemDev::download(nraps_result *r, double ms_total, double ms_device, dou
     d2h(r->draws, dev.draws, 8 * trace_n);
     d2h(r->kacc, dev.trace_kacc, 8 * trace_n);
   }
-  if (stream) HIP_TRY(hipStreamSynchronize(stream));
+  if (cs_stream) HIP_TRY(hipStreamSynchronize(cs_stream));
   auto take = [&](size_t off, size_t n, void *dst) {
     std::memcpy(dst, &acc[off - blk.up_end], n);
   };

@@ -74,7 +74,8 @@ struct nraps_server {
   std::vector<CycleState> h_slots;
   std::vector<int> free_slots;
   std::unique_ptr<Launcher> L;
-  hipStream_t s = nullptr;
+  hipStream_t s = nullptr;   // set-up, cycles and slot read-back
+  hipStream_t s_out = nullptr;  // result read-back of retiring requests
 
   void loop();
   void retire(Request *r, int status);
@@ -109,14 +110,16 @@ void nraps_server::retire(Request *r, int status) {
         o.k_ref = r->kref.data();
         o.sites = r->sites.data();
         double total = ms_between(r->t_submit, Clock::now());
-        int st = r->dev->download(&o, total, 0.0, 0.0, r->cycles_run);
+        // the request's device work has completed; its read-back runs on
+        // its own stream, beside the cycle already queued for the others
+        int st = r->dev->download(&o, total, 0.0, 0.0, r->cycles_run, s_out);
         if (status == NRAPS_OK) status = st;
       }
     } catch (const std::exception &e) {
       status = NRAPS_E_DEVICE;
       set_error(e.what());
     }
-    r->dev.reset();  // memory returns to the stream-ordered pool
+    r->dev.reset();  // memory returns to the pool, ordered after the queued cycle
   }
   std::lock_guard<std::mutex> g(m);
   if (r->slot >= 0) free_slots.push_back(r->slot);
@@ -134,17 +137,21 @@ void nraps_server::retire(Request *r, int status) {
 //   1. admit queued requests (first fit by memory footprint and free slots)
 //      and build their host tables -- while the GPU runs the cycle queued by
 //      the previous iteration;
-//   2. wait for that cycle, read every slot's generation state, retire the
-//      requests that finished (or were cancelled);
+//   2. wait for that cycle, read every slot's generation state, and take
+//      the requests that finished (or were cancelled) out of the resident
+//      set;
 //   3. queue the device set-up of the admitted requests, rebuild the launch
 //      plan if membership changed, and queue the next cycle of every
-//      resident request together with the read-back of the slot states.
-// All device work is ordered on one stream; allocations come from the
-// stream-ordered pool, so set-up and retirement never stall the device.
+//      resident request together with the read-back of the slot states;
+//   4. read back and release the finished requests while that cycle runs.
+// Set-up, cycles and slot read-back are ordered on one stream, result
+// read-back on a second one; allocations come from the stream-ordered pool,
+// so set-up and retirement do not stall the device.
 void nraps_server::loop() {
   try {
     NRAPS_HIP_TRY(hipSetDevice(opts.device));
     NRAPS_HIP_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    NRAPS_HIP_TRY(hipStreamCreateWithFlags(&s_out, hipStreamNonBlocking));
     keep_pool_memory(opts.device);
     L.reset(new Launcher(opts.device));
     bool dirty = true;
@@ -196,6 +203,7 @@ void nraps_server::loop() {
         }
       }
       // the queued cycle: wait, then read the generation state of its slots
+      std::vector<std::pair<Request *, int>> done_list;
       if (!inflight.empty()) {
         NRAPS_HIP_TRY(hipStreamSynchronize(s));
         for (Request *r : inflight) {
@@ -222,8 +230,11 @@ void nraps_server::loop() {
             if (sem <= opts.sigma_k_target * std::fabs(mean)) finished = true;
           }
           if (cancel || finished) {
-            if (cs.active) r->dev->deactivate(s);
-            retire(r, cancel ? NRAPS_E_CANCELLED : NRAPS_OK);
+            // out of the resident set now, so the next launch plan leaves it
+            // out; its results are read back once that cycle is queued
+            std::lock_guard<std::mutex> g(m);
+            resident.erase(std::remove(resident.begin(), resident.end(), r), resident.end());
+            done_list.emplace_back(r, cancel ? NRAPS_E_CANCELLED : NRAPS_OK);
             dirty = true;
           }
         }
@@ -248,7 +259,10 @@ void nraps_server::loop() {
         std::lock_guard<std::mutex> g(m);
         run = resident;
       }
-      if (run.empty()) continue;
+      if (run.empty()) {
+        for (auto &f : done_list) retire(f.first, f.second);
+        continue;
+      }
       if (dirty) {
         std::vector<ProblemDev *> ps;
         for (Request *r : run) ps.push_back(r->dev.get());
@@ -263,6 +277,7 @@ void nraps_server::loop() {
       NRAPS_HIP_TRY(hipMemcpyAsync(h_slots.data(), d_slots, sizeof(CycleState) * h_slots.size(),
                                    hipMemcpyDeviceToHost, s));
       inflight = run;
+      for (auto &f : done_list) retire(f.first, f.second);
     }
   } catch (const std::exception &e) {
     std::lock_guard<std::mutex> g(m);
@@ -278,13 +293,15 @@ void nraps_server::loop() {
     pending.clear();
     cv_done.notify_all();
   }
-  // release device memory of anything still resident (while the stream its
-  // pool allocations are ordered on is alive)
+  // release the device memory of every request still holding some (while
+  // the stream its pool allocations are ordered on is alive)
   if (s) (void)hipStreamSynchronize(s);
+  if (s_out) (void)hipStreamSynchronize(s_out);
   std::vector<Request *> left;
   {
     std::lock_guard<std::mutex> g(m);
-    left = resident;
+    for (auto &kv : reqs)
+      if (kv.second->dev) left.push_back(kv.second.get());
   }
   const bool failed = !fatal.empty();
   for (Request *r : left) {
@@ -301,7 +318,9 @@ void nraps_server::loop() {
     (void)hipStreamSynchronize(s);
     (void)hipStreamDestroy(s);
   }
+  if (s_out) (void)hipStreamDestroy(s_out);
   s = nullptr;
+  s_out = nullptr;
 }
 
 extern "C" int nraps_server_create(const nraps_server_opts *o, nraps_server **out) {
