@@ -268,6 +268,10 @@ int nraps_server_cancel(nraps_server *srv, uint64_t ticket);
 /* Latency of a finished request: submit->admit and submit->done, ms. */
 int nraps_server_latency(nraps_server *srv, uint64_t ticket, double *queued_ms,
                          double *total_ms, int32_t *cycles_run);
+/* Drop a finished request's record and results (the server keeps them until
+   then, so a long-running server should forget what it has read).  Returns
+   NRAPS_PENDING for a request that is not finished. */
+int nraps_server_forget(nraps_server *srv, uint64_t ticket);
 void nraps_server_destroy(nraps_server *srv);
 
 #ifdef __cplusplus

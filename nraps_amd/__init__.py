@@ -63,6 +63,7 @@ def lib():
             "nraps_server_cancel": (C.c_int, [C.c_void_p, C.c_uint64]),
             "nraps_server_latency": (C.c_int, [C.c_void_p, C.c_uint64, C.POINTER(C.c_double),
                                                C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
+            "nraps_server_forget": (C.c_int, [C.c_void_p, C.c_uint64]),
             "nraps_server_destroy": (None, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
@@ -200,6 +201,13 @@ class Server:
         q, t, c = C.c_double(), C.c_double(), C.c_int32()
         st = lib().nraps_server_latency(self._h, ticket, C.byref(q), C.byref(t), C.byref(c))
         return st, q.value, t.value, c.value
+
+    def forget(self, ticket):
+        """Drop a finished request's record (and the arrays kept alive for it)."""
+        st = lib().nraps_server_forget(self._h, ticket)
+        if st == 0:
+            self._keep.pop(ticket, None)
+        return st
 
     def close(self):
         if self._h:

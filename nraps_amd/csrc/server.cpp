@@ -493,6 +493,19 @@ extern "C" int nraps_server_latency(nraps_server *srv, uint64_t ticket, double *
   return NRAPS_OK;
 }
 
+extern "C" int nraps_server_forget(nraps_server *srv, uint64_t ticket) {
+  if (!srv) return NRAPS_E_INVALID;
+  std::lock_guard<std::mutex> g(srv->m);
+  auto it = srv->reqs.find(ticket);
+  if (it == srv->reqs.end()) {
+    set_error("unknown ticket");
+    return NRAPS_E_NOT_FOUND;
+  }
+  if (it->second->state != Request::DONE) return NRAPS_PENDING;
+  srv->reqs.erase(it);
+  return NRAPS_OK;
+}
+
 extern "C" void nraps_server_destroy(nraps_server *srv) {
   if (!srv) return;
   {

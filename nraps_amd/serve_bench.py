@@ -108,6 +108,7 @@ def run_serving(rate, seconds, *, first_index=0, sigma_pcm=50.0, min_active=10,
                 qlat.append(q)
                 hist += r.histories_run
                 cycles.append(cyc)
+            srv.forget(t)
         wall = time.perf_counter() - t0
     return {
         "rate_per_s": rate, "seconds": seconds, "submitted": len(tickets), "completed": done,

@@ -99,12 +99,17 @@ def test_server_cancel(gpu):
         t = srv.submit(p, o)
         time.sleep(0.2)
         assert srv.poll(t).status_code == _abi.NRAPS_PENDING
+        assert srv.forget(t) == _abi.NRAPS_PENDING  # not finished: kept
         srv.cancel(t)
         r = srv.wait(t, timeout_ms=120000)
         assert r.status_code == _abi.NRAPS_E_CANCELLED
         # the server keeps serving after a cancellation
         o.generations = 6
-        assert srv.wait(srv.submit(p, o), timeout_ms=120000).status_code == 0
+        t2 = srv.submit(p, o)
+        assert srv.wait(t2, timeout_ms=120000).status_code == 0
+        # finished requests can be forgotten; the ticket is then unknown
+        assert srv.forget(t2) == 0 and srv.forget(t) == 0
+        assert nr.lib().nraps_server_poll(srv._h, t2, None) == _abi.NRAPS_E_NOT_FOUND
 
 
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 257])
