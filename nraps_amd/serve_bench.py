@@ -5,7 +5,9 @@ PCG32, seed 42) is submitted to one nraps_server for a fixed window.  Each
 request is an uneven slab problem drawn like the batched workload (3..201
 regions of 10 cells, 2/4/8 groups, 10^4..10^6 histories per cycle, fission
 bank, 20 inactive cycles) and stops once its relative k standard error is at
-most 50 pcm (or after 120 cycles).  Reported: requests completed,
+most 50 pcm (or after 120 cycles).  Each cycle of the resident mix is
+replayed from a hipGraph captured whenever the membership changes.
+Reported: requests completed,
 p50/p99 submit-to-result latency, and histories/s over the run.
 
 Batched: the 256 uneven problems solved by one nraps_mc_solve_batch call
@@ -72,12 +74,13 @@ def percentile(xs, q):
 
 def run_serving(rate, seconds, *, first_index=0, sigma_pcm=50.0, min_active=10,
                 budget_bytes=200 * 10**9, max_slots=1024, drain_s=600.0, precision=_abi.FP32,
-                log=None):
+                use_graph=1, log=None):
     times = arrival_times(rate, seconds)
     reqs = [nr.generate(4, first_index + i) for i in range(len(times))]
     lat, qlat, hist, cycles = [], [], 0, []
     with nr.Server(precision=precision, budget_bytes=budget_bytes, max_slots=max_slots,
-                   min_active=min_active, sigma_k_target=sigma_pcm * 1e-5, use_graph=0) as srv:
+                   min_active=min_active, sigma_k_target=sigma_pcm * 1e-5,
+                   use_graph=use_graph) as srv:
         tickets = []
         t0 = time.perf_counter()
         for (p, o), ta in zip(reqs, times):
@@ -116,7 +119,7 @@ def run_serving(rate, seconds, *, first_index=0, sigma_pcm=50.0, min_active=10,
         "queue_p50_ms": percentile(qlat, 0.50), "queue_p99_ms": percentile(qlat, 0.99),
         "histories": hist, "wall_s": wall, "histories_per_s": hist / wall if wall > 0 else None,
         "mean_cycles": sum(cycles) / len(cycles) if cycles else None,
-        "sigma_k_target_pcm": sigma_pcm,
+        "sigma_k_target_pcm": sigma_pcm, "graph": bool(use_graph),
     }
 
 
@@ -146,6 +149,9 @@ def main():
     ap.add_argument("--drain", type=float, default=600.0)
     ap.add_argument("--batch", type=int, default=0, help="run the batched form with n problems")
     ap.add_argument("--generations", type=int, default=0)
+    ap.add_argument("--no-graph", action="store_true",
+                    help="serving: launch each cycle directly instead of replaying a hipGraph "
+                         "captured while membership is unchanged")
     a = ap.parse_args()
 
     def log(m):
@@ -157,7 +163,8 @@ def main():
         return
     idx = 0
     for rate in [float(x) for x in a.rates.split(",") if x]:
-        out = run_serving(rate, a.seconds, first_index=idx, drain_s=a.drain, log=log)
+        out = run_serving(rate, a.seconds, first_index=idx, drain_s=a.drain,
+                          use_graph=0 if a.no_graph else 1, log=log)
         idx += out["submitted"]
         print(json.dumps({"form": "serving", **out}), flush=True)
 

@@ -49,10 +49,16 @@ def test_graph_replay_equals_direct_launches(gpu):
     assert_same_k(a, b)
 
 
-def test_server_results_equal_alone_and_in_the_mix(gpu):
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_server_results_equal_alone_and_in_the_mix(gpu, use_graph):
+    """Continuous batching: 8 requests through 4 slots (later arrivals join a
+    running mix, with and without per-cycle hipGraph capture); each result
+    equals the request solved alone: k bit for bit, fluxes to FP32 atomic
+    summation order."""
     probs = [small_c3(i, histories=15000, generations=10, skip=4) for i in range(8)]
     solo = [nr.solve(p, o) for p, o in probs]
-    with nr.Server(precision=_abi.FP32, budget_bytes=8 * 10**9, max_slots=4) as srv:
+    with nr.Server(precision=_abi.FP32, budget_bytes=8 * 10**9, max_slots=4,
+                   use_graph=use_graph) as srv:
         tickets = []
         for j, (p, o) in enumerate(probs):
             tickets.append(srv.submit(p, o))
