@@ -460,8 +460,9 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     blocks_per_cu_ = woodcock_blocks_per_cu(variant_, lds_bytes_);
     grid_ = n_cu_ * blocks_per_cu_;
     const int64_t waves = (int64_t)grid_ * (variant_.block_size() / 64);
-    int64_t c = total / (waves * 8);
-    // about 8 claims per wave and launch; the cap bounds the tail a last
+    int64_t c = total / (waves * 6);
+    // about 6 claims per wave and launch (measured best between 4 and 8 on
+    // the 9-region and 201-region slabs); the cap bounds the tail a last
     // claim leaves at the end of the launch (forced 1024-history claims are
     // 11% slower than automatic ones on a 10^7-history fission run) while
     // large serving mixes (10^8 histories per launch) still get few atomics
