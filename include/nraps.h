@@ -52,6 +52,7 @@ extern "C" {
 #define NRAPS_E_BANK_OVERFLOW -8   /* fission sites exceeded the bank capacity */
 #define NRAPS_E_NOT_FOUND -9       /* unknown ticket / config */
 #define NRAPS_E_IO -10             /* file could not be read or written */
+#define NRAPS_E_NOT_CONVERGED -11  /* diffusion: iteration limit reached (results kept) */
 
 /* ---- run option enumerations ------------------------------------------- */
 #define NRAPS_FP64 0 /* double tracking and tallies */
@@ -68,6 +69,11 @@ extern "C" {
 #define NRAPS_XS_ON_MATERIAL_CHANGE 0 /* reference: XS index refreshed only on
                                          entry to a new material, src/mc_code.rs:147 */
 #define NRAPS_XS_EVERY_COLLISION 1    /* XS follow the current group */
+
+#define NRAPS_BANK_INIT_UNIFORM_FUEL 0 /* fission bank: generations 0 and 1 start
+                                          from the reference source */
+#define NRAPS_BANK_INIT_DIFFUSION 1    /* fission bank: generation 0 starts from
+                                          the diffusion fission source */
 
 /* ---- problem ----------------------------------------------------------- */
 typedef struct nraps_problem {
@@ -113,7 +119,13 @@ typedef struct nraps_run_opts {
   int64_t trace_histories;
   int32_t trace_events;
   int32_t trace_cycle;
-  int32_t reserved[8];
+  /* fission-bank start (NRAPS_BANK_INIT_*).  With NRAPS_BANK_INIT_DIFFUSION
+     the solver first runs nraps_diffusion_solve on the device and places the
+     N generation-0 sources at the quantiles (p + 1/2)/N of its fission source
+     (piecewise constant per cell); generation 0 then banks sites with
+     1/k_diffusion.  Only with NRAPS_SOURCE_FISSION_BANK. */
+  int32_t bank_init;
+  int32_t reserved[7];
 } nraps_run_opts;
 
 /* One recorded history event (parity traces). */
@@ -192,6 +204,51 @@ const char *nraps_last_error(void);
 /* Library build / device description, e.g. "nraps 1 gfx950 ...". */
 int nraps_device_info(int32_t device, char *buf, int32_t buflen);
 
+/* ---- multigroup diffusion (src/discrete.rs) ----------------------------- */
+/* Finite-difference multigroup diffusion k-eigenvalue by power iteration,
+   the reference's nalgebra_method (src/discrete.rs:169-356), on the device.
+   The reference inverts each group's n x n matrix densely
+   (src/discrete.rs:186-205) and multiplies the inverse with the right-hand
+   side every iteration.  The matrix is tridiagonal: the library factors it
+   once and solves each group in O(n), which gives the same A^-1 rhs.  The
+   matrix rules, the group sweep, the convergence test and the power
+   normalisation of the flux are the reference's. */
+typedef struct nraps_diffusion_opts {
+  int32_t device;
+  int32_t max_iterations; /* 0 => 1000000; hitting it => NRAPS_E_NOT_CONVERGED */
+  double flux_tol;        /* iterate while flux change >= flux_tol (0 => 1e-5) */
+  double k_tol;           /*   and k change >= k_tol (0 => 1e-6), src/discrete.rs:207 */
+  int32_t reserved[8];
+} nraps_diffusion_opts;
+
+typedef struct nraps_diffusion_result {
+  /* caller-allocated outputs; any pointer may be NULL to skip it */
+  double *flux;             /* [G*nc] power-normalised flux (src/discrete.rs:304-343) */
+  double *assembly_average; /* [G*nc] (src/discrete.rs:151-167) */
+  double *fission_source;   /* [nc] sum over groups of nut*sigf*flux */
+  /* scalar outputs */
+  double k;
+  double delta_flux;        /* convergence measures at exit (src/discrete.rs:252-289) */
+  double delta_k;
+  int32_t iterations;
+  int32_t status;           /* same value the call returned */
+  double ms_total;          /* wall time, upload to download */
+  double ms_device;         /* device time of the iteration kernel */
+  int32_t reserved[8];
+} nraps_diffusion_result;
+
+void nraps_diffusion_opts_default(nraps_diffusion_opts *opts);
+
+/* Solve one problem.  Uses n_cells >= 2, the XS of the problem (inv_sigtr,
+   sigt, scat_matrix, nut, sigf, chit), boundl/boundr and numass. */
+int nraps_diffusion_solve(const nraps_problem *problem, const nraps_diffusion_opts *opts,
+                          nraps_diffusion_result *result);
+
+/* Solve n independent problems in one launch (one workgroup per problem). */
+int nraps_diffusion_solve_batch(int32_t n, const nraps_problem *const *problems,
+                                const nraps_diffusion_opts *opts,
+                                nraps_diffusion_result *const *results);
+
 /* ---- PCG32 (src/rand.rs:36-107) ---------------------------------------- */
 /* out[s*draws + i] = i-th next_u32 of stream (stream0 + s) seeded like
    Prn32::new with an explicit seed (src/rand.rs:49-71).  Runs on the device. */
@@ -238,6 +295,14 @@ int nraps_validate(const nraps_problem *problem, const nraps_run_opts *opts);
    layout and number format (src/plot_solution.rs:14-58). */
 int nraps_write_outputs(const char *dir, const nraps_problem *problem,
                         const nraps_run_opts *opts, const nraps_result *result);
+
+/* Write the diffusion solution in the same files: vars.csv (generations is
+   the deck's, as src/main.rs passes it), interface.csv (G flux rows, G
+   assembly rows and the fission-source row) and k_eff.csv (one row, k).
+   The reference hands plot_solution an empty fission source and k_fund for
+   this path (src/discrete.rs:347-355); here the rows are filled. */
+int nraps_write_diffusion_outputs(const char *dir, const nraps_problem *problem,
+                                  int64_t generations, const nraps_diffusion_result *result);
 
 /* ---- serving: continuous batching of solve requests --------------------- */
 typedef struct nraps_server nraps_server;

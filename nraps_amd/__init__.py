@@ -12,11 +12,12 @@ import numpy as np
 
 from . import _abi
 from ._abi import *  # noqa: F401,F403  (constants)
-from .problem import ResultBuffers, SlabProblem, run_opts
+from .problem import DiffusionBuffers, ResultBuffers, SlabProblem, diffusion_opts, run_opts
 
 __all__ = ["lib", "solve", "solve_batch", "generate", "load_deck", "write_outputs",
            "pcg32_fill", "device_info", "validate", "Server", "SlabProblem", "run_opts",
-           "ResultBuffers", "NrapsError"]
+           "ResultBuffers", "NrapsError", "diffusion_solve", "diffusion_solve_batch",
+           "diffusion_opts", "DiffusionBuffers", "write_diffusion_outputs"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 # NRAPS_LIB selects another build of the library (e.g. a tuning variant)
@@ -40,6 +41,7 @@ def lib():
             raise RuntimeError(f"{LIB_PATH} is missing: run `python -m nraps_amd.build`")
         L = C.CDLL(LIB_PATH)
         P, O, R = C.POINTER(_abi.Problem), C.POINTER(_abi.RunOpts), C.POINTER(_abi.Result)
+        DO, DR = C.POINTER(_abi.DiffusionOpts), C.POINTER(_abi.DiffusionResult)
         sig = {
             "nraps_run_opts_default": (None, [O]),
             "nraps_mc_solve": (C.c_int, [P, O, R]),
@@ -65,6 +67,11 @@ def lib():
                                                C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
             "nraps_server_forget": (C.c_int, [C.c_void_p, C.c_uint64]),
             "nraps_server_destroy": (None, [C.c_void_p]),
+            "nraps_write_diffusion_outputs": (C.c_int, [C.c_char_p, P, C.c_int64, DR]),
+            "nraps_diffusion_opts_default": (None, [DO]),
+            "nraps_diffusion_solve": (C.c_int, [P, DO, DR]),
+            "nraps_diffusion_solve_batch": (C.c_int, [C.c_int32, C.POINTER(P), DO,
+                                                      C.POINTER(DR)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -114,6 +121,30 @@ def solve_batch(problems, opts_list, *, check=True):
     return rbs
 
 
+def diffusion_solve(problem, opts=None, *, check=True):
+    """Run nraps_diffusion_solve; returns the DiffusionBuffers."""
+    opts = opts if opts is not None else diffusion_opts()
+    rb = DiffusionBuffers(problem)
+    st = lib().nraps_diffusion_solve(C.byref(problem.view()), C.byref(opts), C.byref(rb.c))
+    rb.status_code = st
+    if check:
+        _check(st)
+    return rb
+
+
+def diffusion_solve_batch(problems, opts=None, *, check=True):
+    opts = opts if opts is not None else diffusion_opts()
+    n = len(problems)
+    views = [p.view() for p in problems]
+    rbs = [DiffusionBuffers(p) for p in problems]
+    pa = (C.POINTER(_abi.Problem) * n)(*[C.pointer(v) for v in views])
+    ra = (C.POINTER(_abi.DiffusionResult) * n)(*[C.pointer(r.c) for r in rbs])
+    st = lib().nraps_diffusion_solve_batch(n, pa, C.byref(opts), ra)
+    if check:
+        _check(st)
+    return rbs
+
+
 def generate(config, index=0, seed=42):
     """Synthetic workload `config` (0..4); returns (SlabProblem, RunOpts)."""
     h = C.c_void_p()
@@ -141,6 +172,11 @@ def load_deck(path):
 def write_outputs(directory, problem, opts, rb):
     _check(lib().nraps_write_outputs(os.fsencode(directory), C.byref(problem.view()),
                                      C.byref(opts), C.byref(rb.c)))
+
+
+def write_diffusion_outputs(directory, problem, generations, db):
+    _check(lib().nraps_write_diffusion_outputs(os.fsencode(directory), C.byref(problem.view()),
+                                               int(generations), C.byref(db.c)))
 
 
 def pcg32_fill(seed, stream0, n_streams, draws):

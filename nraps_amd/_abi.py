@@ -18,11 +18,13 @@ NRAPS_E_NO_FISSION = -7
 NRAPS_E_BANK_OVERFLOW = -8
 NRAPS_E_NOT_FOUND = -9
 NRAPS_E_IO = -10
+NRAPS_E_NOT_CONVERGED = -11
 
 FP64, FP32 = 0, 1
 HISTORY_SURFACE, EVENT_WOODCOCK = 0, 1
 SOURCE_UNIFORM_FUEL, SOURCE_FISSION_BANK = 0, 1
 XS_ON_MATERIAL_CHANGE, XS_EVERY_COLLISION = 0, 1
+BANK_INIT_UNIFORM_FUEL, BANK_INIT_DIFFUSION = 0, 1
 
 EV_BIRTH, EV_ABSORB, EV_SCATTER, EV_LEAK, EV_CAP = 0, 1, 2, 3, 4
 
@@ -50,7 +52,7 @@ class RunOpts(C.Structure):
         ("xs_refresh", C.c_int32), ("k_initial", C.c_double), ("max_events", C.c_int64),
         ("device", C.c_int32), ("use_graph", C.c_int32), ("chunk", C.c_int64),
         ("trace_histories", C.c_int64), ("trace_events", C.c_int32),
-        ("trace_cycle", C.c_int32), ("reserved", C.c_int32 * 8),
+        ("trace_cycle", C.c_int32), ("bank_init", C.c_int32), ("reserved", C.c_int32 * 7),
     ]
 
 
@@ -96,4 +98,20 @@ class ServerOpts(C.Structure):
         ("max_slots", C.c_int32), ("min_active", C.c_int32),
         ("sigma_k_target", C.c_double), ("use_graph", C.c_int32),
         ("reserved", C.c_int32 * 7),
+    ]
+
+
+class DiffusionOpts(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32), ("max_iterations", C.c_int32), ("flux_tol", C.c_double),
+        ("k_tol", C.c_double), ("reserved", C.c_int32 * 8),
+    ]
+
+
+class DiffusionResult(C.Structure):
+    _fields_ = [
+        ("flux", _dp), ("assembly_average", _dp), ("fission_source", _dp),
+        ("k", C.c_double), ("delta_flux", C.c_double), ("delta_k", C.c_double),
+        ("iterations", C.c_int32), ("status", C.c_int32), ("ms_total", C.c_double),
+        ("ms_device", C.c_double), ("reserved", C.c_int32 * 8),
     ]

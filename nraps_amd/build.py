@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libnraps.so")
 CLI = os.path.join(HERE, "nraps_run")
 ARCH = os.environ.get("NRAPS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["host.cpp", "kernels.hip", "runtime.hip", "server.cpp"]
+SOURCES = ["host.cpp", "kernels.hip", "runtime.hip", "server.cpp", "diffusion.hip"]
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
           "-Wall", "-Wno-unused-function", "-I", CSRC]
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-munsafe-fp-atomics",

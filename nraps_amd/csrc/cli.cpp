@@ -1,15 +1,19 @@
-// cli.cpp — command-line driver: input deck -> mesh -> Monte Carlo solve ->
-// reference CSV outputs, the run that src/main.rs performs for the Monte Carlo
-// solution (process_input, mesh_gen, monte_carlo, plot_solution; src/main.rs:
-// 85-143, :337, :358-364), on the GPU through the C ABI.
+// cli.cpp — command-line driver: input deck -> mesh -> solve -> reference CSV
+// outputs, the run of src/main.rs (process_input, mesh_gen, the solution
+// dispatch and plot_solution; src/main.rs:85-143, :335-364), on the GPU
+// through the C ABI.
 //
-//   nraps_run DECK [OUTDIR] [--fp32|--fp64] [--event|--plain] [--bank]
-//             [--histories N] [--generations N] [--skip N] [--seed S]
-//             [--every-collision]
+//   nraps_run DECK [OUTDIR] [--mc|--diffusion] [--fp32|--fp64] [--event|--plain]
+//             [--bank] [--diffusion-start] [--histories N] [--generations N]
+//             [--skip N] [--seed S] [--every-collision]
 //
-// Defaults: the deck's histories/generations/skip, FP32 event-based
-// Woodcock tracking, the reference source and XS rules, seed 42.  Writes
-// vars.csv, interface.csv and k_eff.csv into OUTDIR (default ".").
+// The deck's `Solution` picks the solver as the reference's dispatcher does
+// (src/main.rs:336-346): 1 runs the Monte Carlo solve, otherwise the
+// finite-difference diffusion solve (nalgebra_method) for `solver` 0; the
+// reference implements no other solver.  --mc / --diffusion override it.
+// Monte Carlo defaults: the deck's histories/generations/skip, FP32
+// event-based Woodcock tracking, the reference source and XS rules, seed 42.
+// Writes vars.csv, interface.csv and k_eff.csv into OUTDIR (default ".").
 #include <cerrno>
 #include <cstdio>
 #include <cstdlib>
@@ -23,9 +27,9 @@ namespace {
 
 int usage() {
   std::fprintf(stderr,
-               "usage: nraps_run DECK [OUTDIR] [--fp32|--fp64] [--event|--plain] [--bank]\n"
-               "                 [--histories N] [--generations N] [--skip N] [--seed S]\n"
-               "                 [--every-collision]\n");
+               "usage: nraps_run DECK [OUTDIR] [--mc|--diffusion] [--fp32|--fp64] [--event|--plain]\n"
+               "                 [--bank] [--diffusion-start] [--histories N] [--generations N]\n"
+               "                 [--skip N] [--seed S] [--every-collision]\n");
   return 2;
 }
 
@@ -48,6 +52,7 @@ int main(int argc, char **argv) {
   o.precision = NRAPS_FP32;
   o.method = NRAPS_EVENT_WOODCOCK;
   long long hist = -1, gens = -1, skip = -1, seed = -1;
+  int solver = -1;  // -1: the deck's Solution, 0: Monte Carlo, 1: diffusion
   std::vector<std::string> pos;
   for (int i = 1; i < argc; i++) {
     std::string a = argv[i];
@@ -56,7 +61,10 @@ int main(int argc, char **argv) {
       i++;
       return true;
     };
-    if (a == "--fp32") o.precision = NRAPS_FP32;
+    if (a == "--mc") solver = 0;
+    else if (a == "--diffusion") solver = 1;
+    else if (a == "--diffusion-start") o.bank_init = NRAPS_BANK_INIT_DIFFUSION;
+    else if (a == "--fp32") o.precision = NRAPS_FP32;
     else if (a == "--fp64") o.precision = NRAPS_FP64;
     else if (a == "--event") o.method = NRAPS_EVENT_WOODCOCK;
     else if (a == "--plain") o.method = NRAPS_HISTORY_SURFACE;
@@ -82,6 +90,43 @@ int main(int argc, char **argv) {
     return 1;
   }
   const nraps_problem *p = nraps_problem_view(buf);
+  if (solver < 0) solver = info.solution_mc ? 0 : 1;
+  if (solver == 1) {
+    if (info.solver >= 1 && info.solver <= 3) {
+      std::fprintf(stderr, "nraps_run: solver %d is not implemented by the reference (only 0, "
+                           "the direct solve); use --mc for the Monte Carlo solution\n", info.solver);
+      nraps_problem_free(buf);
+      return 1;
+    }
+    const size_t G = (size_t)p->n_groups, nc = (size_t)p->n_cells;
+    std::vector<double> flux(G * nc), aa(G * nc), fs(nc);
+    nraps_diffusion_result d;
+    std::memset(&d, 0, sizeof d);
+    d.flux = flux.data();
+    d.assembly_average = aa.data();
+    d.fission_source = fs.data();
+    nraps_diffusion_opts dopt;
+    nraps_diffusion_opts_default(&dopt);
+    std::fprintf(stderr, "nraps_run: %s, %d cells, %d groups, diffusion\n", deck.c_str(), p->n_cells,
+                 p->n_groups);
+    st = nraps_diffusion_solve(p, &dopt, &d);
+    if (st != NRAPS_OK && st != NRAPS_E_NOT_CONVERGED) {
+      std::fprintf(stderr, "nraps_run: solve failed (%d): %s\n", st, nraps_last_error());
+      nraps_problem_free(buf);
+      return 1;
+    }
+    // src/discrete.rs:345 prints k with 10 decimals
+    std::printf("%.10f\nk = %.10f  (%d iterations, %.2f ms device, %.1f ms)\n", d.k, d.k,
+                d.iterations, d.ms_device, d.ms_total);
+    st = nraps_write_diffusion_outputs(outdir.c_str(), p, info.generations, &d);
+    nraps_problem_free(buf);
+    if (st != NRAPS_OK) {
+      std::fprintf(stderr, "nraps_run: %s\n", nraps_last_error());
+      return 1;
+    }
+    return 0;
+  }
+  if (o.bank_init == NRAPS_BANK_INIT_DIFFUSION) o.source = NRAPS_SOURCE_FISSION_BANK;
   o.histories = hist > 0 ? hist : info.histories;
   o.generations = (int32_t)(gens > 0 ? gens : info.generations);
   o.skip = (int32_t)(skip > 0 ? skip : info.skip);

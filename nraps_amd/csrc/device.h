@@ -48,6 +48,7 @@ struct DevProblem {
   TabLayout lay;
   int32_t nc, G, M, R, nf;
   int32_t xs_every, fission;
+  int32_t dstart;       // fission bank started from the diffusion source
   int32_t tally_bins;  // 2*G*nc (direct + difference array)
   int32_t nb;          // cell-lookup buckets
   double inv_h;        // 1 / bucket width

@@ -92,7 +92,14 @@ int validate(const nraps_problem *pb, const nraps_run_opts *o, std::string *why)
     return bad("unknown source mode");
   if (o->xs_refresh != NRAPS_XS_ON_MATERIAL_CHANGE && o->xs_refresh != NRAPS_XS_EVERY_COLLISION)
     return bad("unknown xs_refresh");
-  if (o->source == NRAPS_SOURCE_FISSION_BANK && (o->skip < 2 || o->generations < 3))
+  if (o->bank_init != NRAPS_BANK_INIT_UNIFORM_FUEL && o->bank_init != NRAPS_BANK_INIT_DIFFUSION)
+    return bad("unknown bank_init");
+  if (o->bank_init == NRAPS_BANK_INIT_DIFFUSION && o->source != NRAPS_SOURCE_FISSION_BANK)
+    return bad("bank_init = diffusion needs the fission-bank source");
+  if (o->bank_init == NRAPS_BANK_INIT_DIFFUSION && nc < 2)
+    return bad("bank_init = diffusion needs at least 2 cells");
+  if (o->source == NRAPS_SOURCE_FISSION_BANK && o->bank_init == NRAPS_BANK_INIT_UNIFORM_FUEL &&
+      (o->skip < 2 || o->generations < 3))
     return bad("fission-bank mode needs skip >= 2 (two uniform-source start-up generations)");
   if (o->problem_id >= (1ull << 19)) return bad("problem_id must be < 2^19");
   if (!(o->k_initial > 0.0) || !std::isfinite(o->k_initial)) return bad("k_initial must be positive");
@@ -871,6 +878,45 @@ extern "C" int nraps_write_outputs(const char *dir, const nraps_problem *pb, con
     iface += row(r->fission_source, nc);
     write("interface.csv", iface);
     write("k_eff.csv", row(r->k, gens) + row(r->k_fund, gens));
+    return NRAPS_OK;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return NRAPS_E_IO;
+  }
+}
+
+extern "C" int nraps_write_diffusion_outputs(const char *dir, const nraps_problem *pb,
+                                             int64_t generations,
+                                             const nraps_diffusion_result *r) {
+  try {
+    if (!dir || !pb || !r || !r->flux || !r->assembly_average || !r->fission_source ||
+        pb->n_cells < 1) {
+      set_error("nraps_write_diffusion_outputs: missing argument or result array");
+      return NRAPS_E_INVALID;
+    }
+    const int nc = pb->n_cells, G = pb->n_groups;
+    std::string iface;
+    auto row = [&](const double *v) {
+      for (int i = 0; i < nc; i++) {
+        if (i) iface += ',';
+        iface += rust_display(v[i]);
+      }
+      iface += '\n';
+    };
+    for (int g = 0; g < G; g++) row(r->flux + (size_t)g * nc);
+    for (int g = 0; g < G; g++) row(r->assembly_average + (size_t)g * nc);
+    row(r->fission_source);
+    auto write = [&](const char *name, const std::string &body) {
+      std::string path = std::string(dir) + "/" + name;
+      FILE *fp = std::fopen(path.c_str(), "wb");
+      if (!fp) throw std::runtime_error("cannot write " + path);
+      std::fwrite(body.data(), 1, body.size(), fp);
+      std::fclose(fp);
+    };
+    write("vars.csv", rust_display(pb->mesh_right[nc - 1]) + "\n" + std::to_string(nc) + "\n" +
+                          std::to_string(generations) + "\n");
+    write("interface.csv", iface);
+    write("k_eff.csv", rust_display(r->k) + "\n");
     return NRAPS_OK;
   } catch (const std::exception &e) {
     set_error(e.what());

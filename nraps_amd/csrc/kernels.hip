@@ -1143,8 +1143,8 @@ __global__ __launch_bounds__(kBlock) void k_finalize_close(const DevProblem *pro
     csp->k_prev = knew;
     csp->inv_k = 1.0 / knew;
     csp->cycle = next;
-    csp->from_bank = P.fission && next >= 2;
-    csp->make_sites = P.fission && next >= 1;
+    csp->from_bank = P.fission && (P.dstart || next >= 2);
+    csp->make_sites = P.fission && (P.dstart || next >= 1);
     csp->site_count = 0;
     if (P.fission && !(knew > 0.0)) csp->error |= 2;
     if (next >= F.generations) {

@@ -3,10 +3,12 @@
 
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <stdexcept>
 #include <vector>
 
 #include "device.h"
+#include "diffusion.h"
 #include "host.h"
 #include "launch.h"
 
@@ -104,6 +106,8 @@ class ProblemDev {
   uint8_t *d_base = nullptr;
   std::vector<uint8_t> stage;
   DevArena arena;
+  // diffusion solve behind a diffusion-started fission bank
+  std::unique_ptr<DiffusionSet> diff;
 };
 
 // Launch tables and parameters for a set of problems run together.

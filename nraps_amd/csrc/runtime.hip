@@ -161,6 +161,13 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
   std::memset(&cs0, 0, sizeof cs0);
   cs0.k_prev = o.k_initial;
   cs0.active = 1;
+  const bool dstart = fission && o.bank_init == NRAPS_BANK_INIT_DIFFUSION;
+  if (dstart) {
+    // generation 0 is born from the diffusion bank and banks sites; its
+    // 1/k is written on the device once the diffusion k is known
+    cs0.from_bank = 1;
+    cs0.make_sites = 1;
+  }
   stage.assign(B.up_end, 0);
   std::memcpy(&stage[B.tab], blob.data(), blob.size());
   if (!cs_slot) std::memcpy(&stage[B.cs], &cs0, sizeof cs0);
@@ -185,6 +192,7 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
   d.nf = P.nf;
   d.xs_every = o.xs_refresh == NRAPS_XS_EVERY_COLLISION;
   d.fission = fission;
+  d.dstart = dstart ? 1 : 0;
   d.tally_bins = 2 * P.G * P.nc;
   d.nb = P.nb;
   d.inv_h = P.inv_h;
@@ -229,6 +237,23 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
     d.trace_n = trace_n;
     d.trace_ev = o.trace_events;
     d.trace_cycle = o.trace_cycle;
+  }
+
+  if (dstart) {
+    // diffusion k-eigenvalue of the problem, then N generation-0 sources at
+    // the quantiles of its fission source (all on the device, in order on s)
+    int dev = 0, lds_limit = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&lds_limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    nraps_diffusion_opts dopt;
+    nraps_diffusion_opts_default(&dopt);
+    dopt.max_iterations = kDiffusionStartIterations;
+    diff.reset(new DiffusionSet());
+    diff->add(pb, dopt, lds_limit);
+    diff->upload(arena, s);
+    diff->launch(s);
+    double *cdf = arena.alloc<double>(nc);
+    launch_diffusion_bank(fp32, diff->dev_problem(0), nc, N, d.src_x, d.src_cell, d_cs, cdf, s);
   }
 
   DevFinal &f = fin;

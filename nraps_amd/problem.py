@@ -176,3 +176,31 @@ class ResultBuffers:
         """Recorded events of history p (numpy structured array)."""
         n = min(int(self.trace_len[p]), events)
         return self.trace[p * events:p * events + n]
+
+
+def diffusion_opts(**kw):
+    """nraps_diffusion_opts with the reference tolerances, updated from kw."""
+    o = _abi.DiffusionOpts()
+    o.flux_tol = 1e-5
+    o.k_tol = 1e-6
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class DiffusionBuffers:
+    """Caller-allocated arrays of one diffusion solve, plus the ctypes view."""
+
+    def __init__(self, problem):
+        G, nc = problem.n_groups, problem.n_cells
+        self.flux = np.zeros(G * nc)
+        self.assembly_average = np.zeros(G * nc)
+        self.fission_source = np.zeros(nc)
+        r = _abi.DiffusionResult()
+        dp = C.POINTER(C.c_double)
+        for name in ("flux", "assembly_average", "fission_source"):
+            setattr(r, name, getattr(self, name).ctypes.data_as(dp))
+        self.c = r
+
+    def __getattr__(self, name):
+        return getattr(self.__dict__["c"], name)

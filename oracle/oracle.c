@@ -701,7 +701,8 @@ static __thread char g_oerr[256];
 
 static int oracle_solve_impl(const nraps_problem *pb, const nraps_run_opts *o, int threads,
                              nraps_result *res, int32_t bank_cycle, double *bank_x,
-                             int32_t *bank_cell, int64_t *bank_n) {
+                             int32_t *bank_cell, int64_t *bank_n, const double *init_x,
+                             const int32_t *init_cell, double init_k) {
   oprob P;
   if (oprob_init(&P, pb)) {
     snprintf(g_oerr, sizeof g_oerr, "bad problem");
@@ -711,8 +712,12 @@ static int oracle_solve_impl(const nraps_problem *pb, const nraps_run_opts *o, i
   const int64_t N = o->histories;
   const int gens = o->generations, skip = o->skip;
   int fission = o->source == NRAPS_SOURCE_FISSION_BANK;
+  /* diffusion start: generation 0 is born from the given sources */
+  int dstart = fission && o->bank_init == NRAPS_BANK_INIT_DIFFUSION;
   if (N < 1 || gens < 1 || skip < 1 || skip >= gens || P.nf < 1 ||
-      (fission && (skip < 2 || gens < 3))) {
+      (fission && !dstart && (skip < 2 || gens < 3)) ||
+      (dstart && (!init_x || !init_cell || !(init_k > 0.0))) ||
+      (!dstart && o->bank_init != NRAPS_BANK_INIT_UNIFORM_FUEL)) {
     oprob_free(&P);
     return NRAPS_E_INVALID;
   }
@@ -738,6 +743,10 @@ static int oracle_solve_impl(const nraps_problem *pb, const nraps_run_opts *o, i
   if (fission) {
     src_x = malloc(sizeof(double) * N);
     src_cell = malloc(sizeof(int32_t) * N);
+    if (dstart) {
+      memcpy(src_x, init_x, sizeof(double) * N);
+      memcpy(src_cell, init_cell, sizeof(int32_t) * N);
+    }
   }
   int status = NRAPS_OK;
   double k_prev = o->k_initial;
@@ -753,9 +762,9 @@ static int oracle_solve_impl(const nraps_problem *pb, const nraps_run_opts *o, i
     cy.p = &P;
     cy.o = o;
     cy.cycle = cyc;
-    cy.from_bank = fission && cyc >= 2;
-    cy.make_sites = fission && cyc >= 1;
-    cy.inv_k = cy.make_sites ? 1.0 / kk[cyc - 1] : 0.0;
+    cy.from_bank = fission && (dstart || cyc >= 2);
+    cy.make_sites = fission && (dstart || cyc >= 1);
+    cy.inv_k = !cy.make_sites ? 0.0 : cyc == 0 ? 1.0 / init_k : 1.0 / kk[cyc - 1];
     cy.src_x = src_x;
     cy.src_cell = src_cell;
     if (bank_x && cyc == bank_cycle && cy.from_bank) {
@@ -824,7 +833,7 @@ static int oracle_solve_impl(const nraps_problem *pb, const nraps_run_opts *o, i
     k_prev = knew;
     /* fission bank: sites in (history, sequence) order; systematic
        resampling to N sources with one offset from the service stream */
-    if (fission && cyc >= 1) {
+    if (cy.make_sites) {
       if (nsites == 0) {
         status = NRAPS_E_NO_FISSION;
       } else {
@@ -922,12 +931,19 @@ static int oracle_solve_impl(const nraps_problem *pb, const nraps_run_opts *o, i
 
 int oracle_solve(const nraps_problem *pb, const nraps_run_opts *o, int threads,
                  nraps_result *res) {
-  return oracle_solve_impl(pb, o, threads, res, -1, NULL, NULL, NULL);
+  return oracle_solve_impl(pb, o, threads, res, -1, NULL, NULL, NULL, NULL, NULL, 0.0);
+}
+
+int oracle_solve_init(const nraps_problem *pb, const nraps_run_opts *o, int threads,
+                      nraps_result *res, const double *init_x, const int32_t *init_cell,
+                      double init_k) {
+  return oracle_solve_impl(pb, o, threads, res, -1, NULL, NULL, NULL, init_x, init_cell, init_k);
 }
 
 int oracle_solve_banks(const nraps_problem *pb, const nraps_run_opts *o, int threads,
                        nraps_result *res, int32_t bank_cycle, double *bank_x,
                        int32_t *bank_cell, int64_t *bank_n) {
   *bank_n = 0;
-  return oracle_solve_impl(pb, o, threads, res, bank_cycle, bank_x, bank_cell, bank_n);
+  return oracle_solve_impl(pb, o, threads, res, bank_cycle, bank_x, bank_cell, bank_n, NULL,
+                           NULL, 0.0);
 }

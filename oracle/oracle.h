@@ -6,6 +6,7 @@
  * load it.  The product never links or calls it.
  *
  * It restates src/mc_code.rs:1-380 and src/rand.rs:36-107 of the reference
+ * (and, in diffusion.c, the diffusion solver of src/discrete.rs:1-356)
  * with the deviations the product documents (single-draw CDF inversion,
  * exactly N histories, guarded edge draws, 32-bit indices), in plain C and
  * double precision, history by history.  It shares exactly one routine with
@@ -34,6 +35,25 @@ int oracle_solve(const nraps_problem *pb, const nraps_run_opts *opts,
 int oracle_solve_banks(const nraps_problem *pb, const nraps_run_opts *opts,
                        int threads, nraps_result *res, int32_t bank_cycle,
                        double *bank_x, int32_t *bank_cell, int64_t *bank_n);
+
+/* Fission-bank run whose generation 0 is born from the given N sources
+   (opts->bank_init == NRAPS_BANK_INIT_DIFFUSION) and banks sites with
+   1/init_k. */
+int oracle_solve_init(const nraps_problem *pb, const nraps_run_opts *opts, int threads,
+                      nraps_result *res, const double *init_x, const int32_t *init_cell,
+                      double init_k);
+
+/* Multigroup diffusion k-eigenvalue, src/discrete.rs:169-356 (diffusion.c). */
+int oracle_diffusion(const nraps_problem *pb, const nraps_diffusion_opts *opts,
+                     nraps_diffusion_result *res);
+/* The tridiagonal matrix of group g (src/discrete.rs:4-101): sub[c], diag[c],
+   sup[c] are the entries (c, c-1), (c, c), (c, c+1). */
+void oracle_diffusion_matrix(const nraps_problem *pb, int32_t g, double *sub, double *diag,
+                             double *sup);
+/* N generation-0 sources at the quantiles (p + 1/2)/N of a per-cell fission
+   source (weights s_c dx_c). */
+int oracle_diffusion_bank(const nraps_problem *pb, const double *fission_source, int64_t N,
+                          double *x, int32_t *cell);
 
 /* Reference helper functions, restated (for the reference's own KATs). */
 void oracle_hit_boundary(double mu, double start_x, double delta_s, double bound,

@@ -272,8 +272,8 @@ def test_command_line_driver_runs_a_reference_deck(gpu, tmp_path):
                        "nraps_run")
     deck = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference",
                         "TestCaseA.txt")
-    r = subprocess.run([cli, deck, str(tmp_path), "--histories", "20000", "--generations", "8",
-                        "--skip", "2"], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([cli, deck, str(tmp_path), "--mc", "--histories", "20000", "--generations",
+                        "8", "--skip", "2"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     for f in ("vars.csv", "interface.csv", "k_eff.csv"):
         assert (tmp_path / f).exists()
@@ -284,3 +284,21 @@ def test_command_line_driver_runs_a_reference_deck(gpu, tmp_path):
     assert len(k_row) == 8
     assert abs(float(k_row[-1]) - rb.k[-1]) <= 1e-12 * abs(rb.k[-1])
     assert f"{rb.k_mean:.8f}" in r.stdout
+
+
+def test_command_line_driver_diffusion_deck(gpu, tmp_path):
+    """TestCaseA asks for the finite-difference solution (Solution = 0):
+    nraps_run runs the diffusion solver and writes k = 1.26, the deck's
+    analytic multiplication factor, with G flux, G assembly and one
+    fission-source row."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cli = os.path.join(root, "nraps_amd", "nraps_run")
+    deck = os.path.join(root, "tests", "golden", "reference", "TestCaseA.txt")
+    r = subprocess.run([cli, deck, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    k = float((tmp_path / "k_eff.csv").read_text().strip())
+    assert abs(k - 1.26) < 1e-9
+    rows = (tmp_path / "interface.csv").read_text().splitlines()
+    assert len(rows) == 2 * 2 + 1 and all(len(x.split(",")) == 408 for x in rows)
+    assert (tmp_path / "vars.csv").read_text().splitlines()[2] == "100"
