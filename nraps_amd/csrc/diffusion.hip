@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -36,9 +37,9 @@ namespace nraps {
 
 namespace {
 
-constexpr int kDT = kDiffThreads;
-constexpr int kDW = kDT / 64;
-constexpr int kScratch = 32;  // doubles of LDS scratch: scan totals, reductions, flags
+// Doubles of LDS scratch after the tables: scan totals and reductions for up
+// to 16 waves, and the change-measure slots.
+constexpr int kScratch = 128;
 
 #define HIP_TRY NRAPS_HIP_TRY
 
@@ -48,10 +49,11 @@ struct Aff {
 };
 __device__ __forceinline__ Aff compose(Aff p, Aff q) { return Aff{q.a * p.a, q.a * p.b + q.b}; }
 
-// Exclusive scan of one affine map per thread, in thread order (FWD) or in
-// reverse thread order; wtot: 2*kDW doubles of LDS.
-template <bool FWD>
+// Exclusive scan of one affine map per thread of a BS-thread workgroup, in
+// thread order (FWD) or in reverse thread order; wtot: 2*BS/64 doubles of LDS.
+template <bool FWD, bool SYNC_AFTER, int BS>
 __device__ __forceinline__ Aff block_exscan(Aff v, double *wtot) {
+  constexpr int kW = BS / 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   Aff inc = v;
 #pragma unroll
@@ -72,20 +74,21 @@ __device__ __forceinline__ Aff block_exscan(Aff v, double *wtot) {
   if (FWD) {
     for (int u = 0; u < w; u++) pre = compose(pre, Aff{wtot[2 * u], wtot[2 * u + 1]});
   } else {
-    for (int u = kDW - 1; u > w; u--) pre = compose(pre, Aff{wtot[2 * u], wtot[2 * u + 1]});
+    for (int u = kW - 1; u > w; u--) pre = compose(pre, Aff{wtot[2 * u], wtot[2 * u + 1]});
   }
-  __syncthreads();
+  if (SYNC_AFTER) __syncthreads();  // wtot is written again by the next scan
   return compose(pre, ex);
 }
 
-// Sum over the workgroup in a fixed order; red: kDW doubles of LDS.
+// Sum over the workgroup in a fixed order; red: BS/64 doubles of LDS.
+template <int BS>
 __device__ __forceinline__ double block_sum(double v, double *red) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
   __syncthreads();
   double s = 0.0;
-  for (int u = 0; u < kDW; u++) s += red[u];
+  for (int u = 0; u < BS / 64; u++) s += red[u];
   __syncthreads();
   return s;
 }
@@ -96,13 +99,14 @@ struct Tabs {
 
 // q_gen (src/discrete.rs:103-126): fission rate F_i = sum_x nut sigf flux
 // per cell, and the sum over groups and cells of q[g][i] = F_i dx_i chi_g.
+template <int BS>
 __device__ __forceinline__ double qgen(const DiffProblem &P, const Tabs &T, const double *fl,
                                        double *F, double *red) {
-  const int t = threadIdx.x, n = P.n, G = P.G, M = P.M, L = P.L, LT = L * kDT;
+  const int t = threadIdx.x, n = P.n, G = P.G, M = P.M, L = P.L, LT = L * BS;
   double part = 0.0;
   for (int j = 0; j < L; j++) {
     if (t * L + j >= n) break;
-    const int idx = j * kDT + t;
+    const int idx = j * BS + t;
     const int m = P.mat[idx];
     double f = 0.0;
     for (int x = 0; x < G; x++) f += T.nsf[m + M * x] * fl[x * LT + idx];
@@ -110,18 +114,20 @@ __device__ __forceinline__ double qgen(const DiffProblem &P, const Tabs &T, cons
     const double fd = f * P.dx[idx];
     for (int g = 0; g < G; g++) part += fd * T.chi[m + M * g];
   }
-  return block_sum(part, red);
+  return block_sum<BS>(part, red);
 }
 
+template <int BS>
 __device__ __forceinline__ void diffusion_run(const DiffProblem &P, const Tabs &T, double *scr,
                                               double *fl, double *F) {
-  const int t = threadIdx.x, n = P.n, G = P.G, M = P.M, L = P.L, LT = L * kDT;
-  double *wtot = scr, *red = scr + 2 * kDW, *sc = scr + 3 * kDW;
+  constexpr int kW = BS / 64;
+  const int t = threadIdx.x, n = P.n, G = P.G, M = P.M, L = P.L, LT = L * BS;
+  double *wtot = scr, *red = scr + 2 * kW, *sc = scr + 3 * kW;
   for (int j = 0; j < L; j++) {
     if (t * L + j >= n) break;
-    for (int g = 0; g < G; g++) fl[g * LT + j * kDT + t] = 1.0;
+    for (int g = 0; g < G; g++) fl[g * LT + j * BS + t] = 1.0;
   }
-  double sumq = qgen(P, T, fl, F, red);
+  double sumq = qgen<BS>(P, T, fl, F, red);
   double k = 1.0, dflux = 1.0, dk = 1.0;
   int it = 0, status = NRAPS_OK;
   while (dflux >= P.ftol && dk >= P.ktol) {
@@ -144,7 +150,7 @@ __device__ __forceinline__ void diffusion_run(const DiffProblem &P, const Tabs &
       Aff acc{1.0, 0.0};
       for (int j = 0; j < L; j++) {
         if (t * L + j >= n) break;
-        const int idx = j * kDT + t;
+        const int idx = j * BS + t;
         const int m = P.mat[idx];
         const double d = P.dx[idx];
         double s = 0.0;
@@ -155,23 +161,23 @@ __device__ __forceinline__ void diffusion_run(const DiffProblem &P, const Tabs &
         fg[idx] = beta;
         acc = compose(acc, Aff{al[idx], beta});
       }
-      double y = block_exscan<true>(acc, wtot).b;
+      double y = block_exscan<true, true, BS>(acc, wtot).b;
       for (int j = 0; j < L; j++) {
         if (t * L + j >= n) break;
-        const int idx = j * kDT + t;
+        const int idx = j * BS + t;
         y = al[idx] * y + fg[idx];
         fg[idx] = y;
       }
       acc = Aff{1.0, 0.0};
       for (int j = L - 1; j >= 0; j--) {
         if (t * L + j >= n) continue;
-        const int idx = j * kDT + t;
+        const int idx = j * BS + t;
         acc = compose(acc, Aff{ga[idx], fg[idx]});
       }
-      double x = block_exscan<false>(acc, wtot).b;
+      double x = block_exscan<false, true, BS>(acc, wtot).b;
       for (int j = L - 1; j >= 0; j--) {
         if (t * L + j >= n) continue;
-        const int idx = j * kDT + t;
+        const int idx = j * BS + t;
         x = ga[idx] * x + fg[idx];
         fg[idx] = x;
       }
@@ -187,7 +193,7 @@ __device__ __forceinline__ void diffusion_run(const DiffProblem &P, const Tabs &
       __syncthreads();
       dflux = sc[3];
     }
-    sumq = qgen(P, T, fl, F, red);
+    sumq = qgen<BS>(P, T, fl, F, red);
     const double kold = k;
     k = kold * (sumq / sq_old);
     dk = fabs((k - kold) / kold);
@@ -198,17 +204,17 @@ __device__ __forceinline__ void diffusion_run(const DiffProblem &P, const Tabs &
   for (int j = 0; j < L; j++) {
     const int i = t * L + j;
     if (i >= n) break;
-    const int idx = j * kDT + t;
+    const int idx = j * BS + t;
     const int m = P.mat[idx];
     for (int g = 0; g < G; g++)
       part += fl[g * LT + idx] * T.nut[m + M * g] * T.sigf[m + M * g] * P.dx_nat[(g * n + i) / G];
   }
-  const double ptot = block_sum(part, red);
+  const double ptot = block_sum<BS>(part, red);
   const double pc = 3565e6 / (1.6022e-13 * 200.0 * ptot);
   for (int j = 0; j < L; j++) {
     const int i = t * L + j;
     if (i >= n) break;
-    const int idx = j * kDT + t;
+    const int idx = j * BS + t;
     const int m = P.mat[idx];
     double fs = 0.0;
     for (int g = 0; g < G; g++) {
@@ -227,26 +233,211 @@ __device__ __forceinline__ void diffusion_run(const DiffProblem &P, const Tabs &
   }
 }
 
-__global__ __launch_bounds__(kDT) void k_diffusion(const DiffProblem *probs) {
+// Register-resident form for G <= GM groups and L <= LM cells per thread:
+// each thread keeps its cells' flux, fission rate, material and width in
+// registers for the whole solve, so LDS carries only the tables and the
+// scan/reduction scratch.  Same arithmetic, in the same order, as
+// diffusion_run.  Per group: loads of the group's factors (independent),
+// the right-hand side, two scans and one barrier for the change measure;
+// the scan totals and the change-measure slots alternate between two
+// buffers so no barrier is needed to recycle them.
+template <int GM, int LM, int BS>
+__device__ __forceinline__ void diffusion_regs(const DiffProblem &P, const Tabs &T, double *scr) {
+  constexpr int kW = BS / 64;
+  const int t = threadIdx.x, n = P.n, G = P.G, M = P.M, L = P.L, LT = L * BS;
+  double *wtA = scr, *wtB = scr + 2 * kW, *redA = scr + 4 * kW, *redB = scr + 5 * kW;
+  double *scs = scr + 6 * kW;  // [2][6]: old and new flux at cells 0, n-2, n-1
+  // the thread's cell widths, fission rates and materials: LDS slots
+  // [LM][256] of this thread only (registers stay free for the flux)
+  double *dx = scr + kScratch, *F = dx + LM * BS;
+  int *mat = reinterpret_cast<int *>(F + LM * BS);
+  const int jn = max(0, min(L, n - t * L));
+  const int j0 = P.pos0 / BS, j1 = P.pos1 / BS, j2 = P.pos2 / BS;
+  double phi[GM][LM];
+#pragma unroll
+  for (int j = 0; j < LM; j++) {
+    const bool on = j < jn;
+    mat[j * BS + t] = on ? P.mat[j * BS + t] : 0;
+    dx[j * BS + t] = on ? P.dx[j * BS + t] : 0.0;
+#pragma unroll
+    for (int g = 0; g < GM; g++) phi[g][j] = 1.0;
+  }
+  auto qgen = [&]() {
+    double part = 0.0;
+#pragma unroll
+    for (int j = 0; j < LM; j++) {
+      if (j < jn) {
+        const int m = mat[j * BS + t];
+        double f = 0.0;
+#pragma unroll
+        for (int x = 0; x < GM; x++)
+          if (x < G) f += T.nsf[m + M * x] * phi[x][j];
+        F[j * BS + t] = f;
+        const double fd = f * dx[j * BS + t];
+#pragma unroll
+        for (int g = 0; g < GM; g++)
+          if (g < G) part += fd * T.chi[m + M * g];
+      }
+    }
+    return block_sum<BS>(part, redA);
+  };
+  double sumq = qgen();
+  double k = 1.0, dflux = 1.0, dk = 1.0;
+  int it = 0, status = NRAPS_OK;
+  while (dflux >= P.ftol && dk >= P.ktol) {
+    if (it >= P.max_it) {
+      status = NRAPS_E_NOT_CONVERGED;
+      break;
+    }
+    it++;
+    const double sq_old = sumq, invk = 1.0 / k;
+#pragma unroll
+    for (int g = 0; g < GM; g++) {
+      if (g >= G) break;
+      const size_t go = (size_t)g * LT;
+      double fa[LM], fm[LM], fc[LM], v[LM];
+#pragma unroll
+      for (int j = 0; j < LM; j++) {
+        if (j < jn) {
+          fa[j] = P.alpha[go + j * BS + t];
+          fm[j] = P.mdiag[go + j * BS + t];
+          fc[j] = P.gamma[go + j * BS + t];
+        }
+      }
+      double *sc = scs + 6 * (g & 1);
+#pragma unroll
+      for (int j = 0; j < LM; j++) {
+        if (t == P.own0 && j == j0) sc[0] = phi[g][j];
+        if (t == P.own1 && j == j1) sc[1] = phi[g][j];
+        if (t == P.own2 && j == j2) sc[2] = phi[g][j];
+      }
+      Aff acc{1.0, 0.0};
+#pragma unroll
+      for (int j = 0; j < LM; j++) {
+        if (j < jn) {
+          const int m = mat[j * BS + t];
+          const double d = dx[j * BS + t];
+          double sct = 0.0;
+#pragma unroll
+          for (int e = 0; e < GM; e++)
+            if (e < G && e != g) sct += T.scat[G * G * m + G * e + g] * phi[e][j] * d;
+          const double rhs = F[j * BS + t] * d * T.chi[m + M * g] * invk + sct;
+          v[j] = rhs * fm[j];
+          acc = compose(acc, Aff{fa[j], v[j]});
+        }
+      }
+      double y = block_exscan<true, false, BS>(acc, (g & 1) ? wtB : wtA).b;
+#pragma unroll
+      for (int j = 0; j < LM; j++) {
+        if (j < jn) {
+          y = fa[j] * y + v[j];
+          v[j] = y;
+        }
+      }
+      acc = Aff{1.0, 0.0};
+#pragma unroll
+      for (int j = LM - 1; j >= 0; j--)
+        if (j < jn) acc = compose(acc, Aff{fc[j], v[j]});
+      double x = block_exscan<false, false, BS>(acc, (g & 1) ? wtA : wtB).b;
+#pragma unroll
+      for (int j = LM - 1; j >= 0; j--) {
+        if (j < jn) {
+          x = fc[j] * x + v[j];
+          phi[g][j] = x;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < LM; j++) {
+        if (t == P.own0 && j == j0) sc[3] = phi[g][j];
+        if (t == P.own1 && j == j1) sc[4] = phi[g][j];
+        if (t == P.own2 && j == j2) sc[5] = phi[g][j];
+      }
+      __syncthreads();
+      // src/discrete.rs:252-283, evaluated by every thread
+      const double new0 = sc[3];
+      double dfl = fmax(fabs((sc[0] - new0) / sc[0]), dflux);
+      if (n >= 3) dfl = fabs((sc[1] - sc[4]) / new0);
+      dflux = fmax(fabs((sc[2] - sc[5]) / new0), dfl);
+    }
+    sumq = qgen();
+    const double kold = k;
+    k = kold * (sumq / sq_old);
+    dk = fabs((k - kold) / kold);
+  }
+  double part = 0.0;
+#pragma unroll
+  for (int j = 0; j < LM; j++) {
+    if (j < jn) {
+      const int i = t * L + j, m = mat[j * BS + t];
+#pragma unroll
+      for (int g = 0; g < GM; g++)
+        if (g < G) part += phi[g][j] * T.nut[m + M * g] * T.sigf[m + M * g] * P.dx_nat[(g * n + i) / G];
+    }
+  }
+  const double ptot = block_sum<BS>(part, redB);
+  const double pc = 3565e6 / (1.6022e-13 * 200.0 * ptot);
+#pragma unroll
+  for (int j = 0; j < LM; j++) {
+    if (j < jn) {
+      const int i = t * L + j, m = mat[j * BS + t];
+      double fs = 0.0;
+#pragma unroll
+      for (int g = 0; g < GM; g++) {
+        if (g < G) {
+          const double val = phi[g][j] * pc;
+          P.flux_out[(size_t)g * n + i] = val;
+          fs += T.nsf[m + M * g] * val;
+        }
+      }
+      P.fs_out[i] = fs;
+    }
+  }
+  if (t == 0) {
+    P.scal[0] = k;
+    P.scal[1] = dflux;
+    P.scal[2] = dk;
+    P.scal[3] = (double)it;
+    P.scal[4] = (double)status;
+  }
+}
+
+// Tables to LDS; returns the scratch that follows them.
+template <int BS>
+__device__ __forceinline__ double *load_tables(const DiffProblem &P, double *sm, Tabs &T) {
+  const int G = P.G, M = P.M, GM = G * M, GGM = G * G * M;
+  for (int i = threadIdx.x; i < GM; i += BS) {
+    sm[i] = P.nsf[i];
+    sm[GM + i] = P.chi[i];
+    sm[2 * GM + i] = P.nut[i];
+    sm[3 * GM + i] = P.sigf[i];
+  }
+  for (int i = threadIdx.x; i < GGM; i += BS) sm[4 * GM + i] = P.scat[i];
+  T = Tabs{sm, sm + GM, sm + 2 * GM, sm + 3 * GM, sm + 4 * GM};
+  __syncthreads();
+  return sm + 4 * GM + GGM;
+}
+
+template <int GM, int LM, int BS>
+__global__ __launch_bounds__(BS) void k_diffusion_regs(const DiffProblem *probs) {
   extern __shared__ double sm[];
   const DiffProblem &P = probs[blockIdx.x];
-  const int G = P.G, M = P.M, GM = G * M, GGM = G * G * M;
-  double *tab = sm;
-  for (int i = threadIdx.x; i < GM; i += kDT) {
-    tab[i] = P.nsf[i];
-    tab[GM + i] = P.chi[i];
-    tab[2 * GM + i] = P.nut[i];
-    tab[3 * GM + i] = P.sigf[i];
-  }
-  for (int i = threadIdx.x; i < GGM; i += kDT) tab[4 * GM + i] = P.scat[i];
-  const Tabs T{tab, tab + GM, tab + 2 * GM, tab + 3 * GM, tab + 4 * GM};
-  double *scr = tab + 4 * GM + GGM;
-  __syncthreads();
+  Tabs T;
+  double *scr = load_tables<BS>(P, sm, T);
+  diffusion_regs<GM, LM, BS>(P, T, scr);
+}
+
+template <int BS>
+__global__ __launch_bounds__(BS) void k_diffusion(const DiffProblem *probs) {
+  extern __shared__ double sm[];
+  const DiffProblem &P = probs[blockIdx.x];
+  const int G = P.G;
+  Tabs T;
+  double *scr = load_tables<BS>(P, sm, T);
   if (P.in_lds) {
     double *fl = scr + kScratch;
-    diffusion_run(P, T, scr, fl, fl + (size_t)G * P.L * kDT);
+    diffusion_run<BS>(P, T, scr, fl, fl + (size_t)G * P.L * BS);
   } else {
-    diffusion_run(P, T, scr, P.gflux, P.gF);
+    diffusion_run<BS>(P, T, scr, P.gflux, P.gF);
   }
 }
 
@@ -339,26 +530,18 @@ void DiffusionSet::add(const nraps_problem *pb, const nraps_diffusion_opts &o, i
   h.pb = pb;
   DiffProblem &d = h.d;
   std::memset(&d, 0, sizeof d);
-  const int L = (n + kDT - 1) / kDT, LT = L * kDT;
   d.n = n;
   d.G = G;
   d.M = M;
-  d.L = L;
   d.max_it = o.max_iterations > 0 ? o.max_iterations : 1000000;
   d.ftol = o.flux_tol > 0 ? o.flux_tol : 1e-5;
   d.ktol = o.k_tol > 0 ? o.k_tol : 1e-6;
-  auto pos = [&](int i) { return (i % L) * kDT + i / L; };
-  d.pos0 = pos(0);
-  d.pos1 = pos(n - 2);
-  d.pos2 = pos(n - 1);
-  d.own0 = 0;
-  d.own1 = (n - 2) / L;
-  d.own2 = (n - 1) / L;
 
-  // group matrices (matrix_gen, src/discrete.rs:4-101) and their factors
-  h.alpha.assign((size_t)G * LT, 0.0);
-  h.mdiag.assign((size_t)G * LT, 0.0);
-  h.gamma.assign((size_t)G * LT, 0.0);
+  // group matrices (matrix_gen, src/discrete.rs:4-101) and their factors,
+  // in cell order (the thread-major layout is made at upload)
+  h.alpha.assign((size_t)G * n, 0.0);
+  h.mdiag.assign((size_t)G * n, 0.0);
+  h.gamma.assign((size_t)G * n, 0.0);
   std::vector<double> sub(n), diag(n), sup(n);
   for (int g = 0; g < G; g++) {
     const double d0 = d_over_dx(pb, ist, 0, g), d1 = d_over_dx(pb, ist, 1, g);
@@ -386,18 +569,12 @@ void DiffusionSet::add(const nraps_problem *pb, const nraps_diffusion_opts &o, i
       if (!std::isfinite(m) || !std::isfinite(w))
         throw std::invalid_argument("diffusion matrix of group " + std::to_string(g) +
                                     " is singular");
-      const size_t at = (size_t)g * LT + pos(c);
+      const size_t at = (size_t)g * n + c;
       h.alpha[at] = -sub[c] * m;
       h.mdiag[at] = m;
       h.gamma[at] = -w;
       wprev = w;
     }
-  }
-  h.mat.assign(LT, 0);
-  h.dx.assign(LT, 0.0);
-  for (int c = 0; c < n; c++) {
-    h.mat[pos(c)] = pb->matid[c];
-    h.dx[pos(c)] = pb->delta_x[c];
   }
   h.nsf.resize(GM);
   h.chi.assign(pb->chit, pb->chit + GM);
@@ -406,14 +583,71 @@ void DiffusionSet::add(const nraps_problem *pb, const nraps_diffusion_opts &o, i
   for (int i = 0; i < GM; i++) h.nsf[i] = pb->nut[i] * pb->sigf[i];
 
   const size_t tab_bytes = ((size_t)4 * GM + (size_t)G * G * M + kScratch) * sizeof(double);
-  const size_t dyn_bytes = ((size_t)G + 1) * LT * sizeof(double);
   if (tab_bytes > (size_t)lds_limit) throw std::invalid_argument("diffusion tables exceed LDS");
-  d.in_lds = tab_bytes + dyn_bytes <= (size_t)lds_limit;
-  lds_bytes_ = std::max(lds_bytes_, tab_bytes + (d.in_lds ? dyn_bytes : 0));
+  lds_limit_ = lds_limit;
   host_.push_back(std::move(h));
 }
 
+// Workgroup size of one problem: 1024 threads (4 waves per SIMD, 1-4 cells
+// per thread) for meshes of 513..4096 cells with at most 8 groups, else 256.
+// It depends on the problem alone, so a result does not depend on the batch
+// it is solved in.
+static int pick_threads(int n, int G) { return (G <= 8 && n > 512 && n <= 4096) ? 1024 : 256; }
+
+void DiffusionSet::layout() {
+  const int np = (int)host_.size();
+  launches_.clear();
+  slot_.assign(np, 0);
+  for (int bs : {256, 1024}) {
+    Launch ln;
+    ln.bs = bs;
+    ln.first = launches_.empty() ? 0 : launches_.back().first + launches_.back().count;
+    for (int i = 0; i < np; i++) {
+      Host &h = host_[i];
+      DiffProblem &d = h.d;
+      if (pick_threads(d.n, d.G) != bs) continue;
+      slot_[i] = ln.first + ln.count++;
+      const int n = d.n, G = d.G;
+      const int L = (n + bs - 1) / bs, LT = L * bs;
+      auto pos = [&](int c) { return (c % L) * bs + c / L; };
+      d.L = L;
+      d.bs = bs;
+      d.pos0 = pos(0);
+      d.pos1 = pos(n - 2);
+      d.pos2 = pos(n - 1);
+      d.own0 = 0;
+      d.own1 = (n - 2) / L;
+      d.own2 = (n - 1) / L;
+      h.p_alpha.assign((size_t)G * LT, 0.0);
+      h.p_mdiag.assign((size_t)G * LT, 0.0);
+      h.p_gamma.assign((size_t)G * LT, 0.0);
+      for (int g = 0; g < G; g++)
+        for (int c = 0; c < n; c++) {
+          h.p_alpha[(size_t)g * LT + pos(c)] = h.alpha[(size_t)g * n + c];
+          h.p_mdiag[(size_t)g * LT + pos(c)] = h.mdiag[(size_t)g * n + c];
+          h.p_gamma[(size_t)g * LT + pos(c)] = h.gamma[(size_t)g * n + c];
+        }
+      h.mat.assign(LT, 0);
+      h.dx.assign(LT, 0.0);
+      for (int c = 0; c < n; c++) {
+        h.mat[pos(c)] = h.pb->matid[c];
+        h.dx[pos(c)] = h.pb->delta_x[c];
+      }
+      const size_t tab_bytes =
+          ((size_t)4 * G * d.M + (size_t)G * G * d.M + kScratch) * sizeof(double);
+      const size_t dyn_bytes = ((size_t)G + 1) * LT * sizeof(double);
+      d.in_lds = tab_bytes + dyn_bytes <= (size_t)lds_limit_;
+      ln.lds_generic = std::max(ln.lds_generic, tab_bytes + (d.in_lds ? dyn_bytes : 0));
+      ln.tab_bytes = std::max(ln.tab_bytes, tab_bytes);
+      ln.max_G = std::max(ln.max_G, G);
+      ln.max_L = std::max(ln.max_L, L);
+    }
+    launches_.push_back(ln);
+  }
+}
+
 void DiffusionSet::upload(DevArena &arena, hipStream_t s) {
+  layout();
   const int np = (int)host_.size();
   // byte layout: [DiffProblem x np][per problem: uploaded arrays][outputs, fallbacks]
   size_t at = 0;
@@ -430,7 +664,7 @@ void DiffusionSet::upload(DevArena &arena, hipStream_t s) {
   const size_t probs = put(sizeof(DiffProblem) * np);
   for (int i = 0; i < np; i++) {
     const Host &h = host_[i];
-    const size_t n = h.d.n, G = h.d.G, GM = G * h.d.M, LT = (size_t)h.d.L * kDT;
+    const size_t n = h.d.n, G = h.d.G, GM = G * h.d.M, LT = (size_t)h.d.L * h.d.bs;
     Off &f = off[i];
     f.alpha = put(G * LT * 8);
     f.mdiag = put(G * LT * 8);
@@ -449,7 +683,7 @@ void DiffusionSet::upload(DevArena &arena, hipStream_t s) {
   const size_t up_end = at;
   for (int i = 0; i < np; i++) {
     const Host &h = host_[i];
-    const size_t n = h.d.n, G = h.d.G, LT = (size_t)h.d.L * kDT;
+    const size_t n = h.d.n, G = h.d.G, LT = (size_t)h.d.L * h.d.bs;
     Off &f = off[i];
     f.gflux = h.d.in_lds ? 0 : put(G * LT * 8);
     f.gF = h.d.in_lds ? 0 : put(LT * 8);
@@ -466,9 +700,9 @@ void DiffusionSet::upload(DevArena &arena, hipStream_t s) {
     const size_t n = h.d.n, G = h.d.G, GM = G * h.d.M;
     const Off &f = off[i];
     auto cp = [&](size_t o, const void *src, size_t bytes) { std::memcpy(&stage[o], src, bytes); };
-    cp(f.alpha, h.alpha.data(), h.alpha.size() * 8);
-    cp(f.mdiag, h.mdiag.data(), h.mdiag.size() * 8);
-    cp(f.gamma, h.gamma.data(), h.gamma.size() * 8);
+    cp(f.alpha, h.p_alpha.data(), h.p_alpha.size() * 8);
+    cp(f.mdiag, h.p_mdiag.data(), h.p_mdiag.size() * 8);
+    cp(f.gamma, h.p_gamma.data(), h.p_gamma.size() * 8);
     cp(f.mat, h.mat.data(), h.mat.size() * 4);
     cp(f.dx, h.dx.data(), h.dx.size() * 8);
     cp(f.dxn, pb->delta_x, n * 8);
@@ -500,7 +734,9 @@ void DiffusionSet::upload(DevArena &arena, hipStream_t s) {
     d.scal = reinterpret_cast<double *>(base + f.scal);
     h_probs_[i] = d;
   }
-  std::memcpy(&stage[probs], h_probs_.data(), sizeof(DiffProblem) * np);
+  // device slots are grouped by workgroup size (one launch per size)
+  for (int i = 0; i < np; i++)
+    std::memcpy(&stage[probs + sizeof(DiffProblem) * slot_[i]], &h_probs_[i], sizeof(DiffProblem));
   d_probs_ = reinterpret_cast<DiffProblem *>(base + probs);
   if (s) {
     // the staging vector must outlive the copy: keep it until the set dies
@@ -511,12 +747,41 @@ void DiffusionSet::upload(DevArena &arena, hipStream_t s) {
   }
 }
 
+namespace {
+using DiffKernel = void (*)(const DiffProblem *);
+
+int regs_lm(int L) { return L <= 2 ? 2 : L <= 4 ? 4 : 8; }
+
+// Register-resident instantiation for the set's workgroup size, largest G
+// (rounded up to 2, 4 or 8) and cells per thread, or nullptr.
+DiffKernel regs_kernel(int bs, int G, int L) {
+  if (G > 8 || (bs == 256 && L > 2) || (bs == 1024 && L > 4)) return nullptr;
+  const int gm = G <= 2 ? 2 : G <= 4 ? 4 : 8, lm = regs_lm(L);
+#define NRAPS_DK(a, b, c) \
+  if (gm == a && lm == b && bs == c) return &k_diffusion_regs<a, b, c>;
+  NRAPS_DK(2, 2, 256) NRAPS_DK(4, 2, 256) NRAPS_DK(8, 2, 256)
+  NRAPS_DK(2, 2, 1024) NRAPS_DK(4, 2, 1024) NRAPS_DK(8, 2, 1024)
+  NRAPS_DK(2, 4, 1024) NRAPS_DK(4, 4, 1024) NRAPS_DK(8, 4, 1024)
+#undef NRAPS_DK
+  return nullptr;
+}
+}  // namespace
+
 void DiffusionSet::launch(hipStream_t s) {
-  if (lds_bytes_ > 65536)
-    HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_diffusion),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes_));
-  hipLaunchKernelGGL(k_diffusion, dim3(host_.size()), dim3(kDT), lds_bytes_, s, d_probs_);
-  HIP_TRY(hipGetLastError());
+  // NRAPS_DIFFUSION_GENERIC=1 forces the LDS/global-memory form (A/B checks)
+  const char *env = std::getenv("NRAPS_DIFFUSION_GENERIC");
+  const bool generic = env && std::atoi(env) != 0;
+  for (const Launch &ln : launches_) {
+    if (ln.count == 0) continue;
+    DiffKernel kr = generic ? nullptr : regs_kernel(ln.bs, ln.max_G, ln.max_L);
+    DiffKernel kern = kr ? kr : ln.bs == 1024 ? &k_diffusion<1024> : &k_diffusion<256>;
+    const size_t lds = kr ? ln.tab_bytes + (size_t)regs_lm(ln.max_L) * ln.bs * 20 : ln.lds_generic;
+    if (lds > 65536)
+      HIP_TRY(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(ln.count), dim3(ln.bs), lds, s, d_probs_ + ln.first);
+    HIP_TRY(hipGetLastError());
+  }
 }
 
 int DiffusionSet::download(int i, nraps_diffusion_result *r, hipStream_t s) {

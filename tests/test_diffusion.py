@@ -280,6 +280,21 @@ def test_device_diffusion_batch_equals_single(gpu):
 
 
 @pytest.mark.gpu
+def test_register_and_lds_forms_identical(gpu, monkeypatch):
+    """The register-resident kernel (G, cells per thread <= 8) and the
+    LDS/global-memory kernel do the same arithmetic in the same order."""
+    probs = [nr.generate(0)[0], nr.generate(2)[0], _deck("TestCaseC"),
+             random_slab(8, bl=0.5, br=0.8), one_group_slab(n=3, bound=0.3)]
+    monkeypatch.setenv("NRAPS_DIFFUSION_GENERIC", "1")
+    lds = nr.diffusion_solve_batch(probs)
+    monkeypatch.delenv("NRAPS_DIFFUSION_GENERIC")
+    regs = nr.diffusion_solve_batch(probs)
+    for a, b in zip(lds, regs):
+        assert a.k == b.k and a.iterations == b.iterations and a.delta_flux == b.delta_flux
+        assert np.array_equal(a.flux, b.flux) and np.array_equal(a.fission_source, b.fission_source)
+
+
+@pytest.mark.gpu
 def test_device_diffusion_iteration_limit(gpu, oracle):
     p = random_slab(4)
     o = diffusion_opts(max_iterations=3, **TIGHT)
