@@ -45,6 +45,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving latency leg")
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
+    ap.add_argument("--no-diffusion", action="store_true", help="skip the diffusion leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -137,6 +138,27 @@ def main():
         log(f"serving at {args.serve_rate:g} req/s: p50 {sv['p50_ms']:.1f} ms, "
             f"p99 {sv['p99_ms']:.1f} ms")
 
+    if not args.no_diffusion:
+        # diffusion solver (src/discrete.rs): the 201-region 8-group slab
+        # (2000 cells) alone, and the 256 uneven slabs in one launch
+        pd = nr.generate(2)[0]
+        nr.diffusion_solve(pd)
+        d1 = [nr.diffusion_solve(pd) for _ in range(5)]
+        dms = sorted(r.ms_device for r in d1)[2]
+        batch = [nr.generate(3, i)[0] for i in range(256)]
+        nr.diffusion_solve_batch(batch[:4])
+        db = [nr.diffusion_solve_batch(batch) for _ in range(3)]
+        bms = sorted(r[0].ms_device for r in db)[1]
+        its = d1[0].iterations
+        out["diffusion"] = {
+            "workload": "201-region 8-group slab, 2000 cells: power iteration to the "
+                        "reference's tolerances", "iterations": its, "k": d1[0].k,
+            "ms_device": dms, "us_per_group_solve": dms * 1e3 / (its * pd.n_groups),
+            "bound": "latency (8 sequential group solves per iteration, 2 block scans each)",
+            "batch_problems": 256, "batch_ms_device": bms,
+            "batch_problems_per_s": 256 / (bms * 1e-3)}
+        log(f"diffusion: {its} iterations in {dms:.2f} ms; 256-problem batch {bms:.2f} ms")
+
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib
@@ -154,6 +176,22 @@ def main():
                                          f"{args.cpu_generations} generations, {wc:.1f} s"}
         out["speedup_vs_cpu"] = value / cpu_v
         log(f"CPU oracle: {cpu_v:.4g} histories/s on {threads} threads")
+        if "diffusion" in out:
+            # the FP64 CPU oracle of the diffusion solver (one core, O(n) solves)
+            t = time.perf_counter()
+            oracle_lib.diffusion(pd)
+            c1 = (time.perf_counter() - t) * 1e3
+            t = time.perf_counter()
+            for q in batch[:32]:
+                oracle_lib.diffusion(q)
+            cb = (time.perf_counter() - t) * 1e3 * 256 / 32
+            out["diffusion"]["cpu_baseline"] = {
+                "ms": c1, "batch_ms": cb, "cores": 1, "kind": "port",
+                "sample": "FP64 CPU oracle (tridiagonal solves): the 2000-cell slab once, "
+                          "32 of the 256 batch problems scaled to 256"}
+            out["diffusion"]["speedup_vs_cpu"] = c1 / out["diffusion"]["ms_device"]
+            out["diffusion"]["batch_speedup_vs_cpu"] = cb / out["diffusion"]["batch_ms_device"]
+            log(f"CPU diffusion oracle: {c1:.1f} ms; batch {cb:.0f} ms")
     print(json.dumps(out), flush=True)
 
 

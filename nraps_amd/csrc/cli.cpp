@@ -8,7 +8,7 @@
 //             [--skip N] [--seed S] [--every-collision]
 //
 // The deck's `Solution` picks the solver as the reference's dispatcher does
-// (src/main.rs:336-346): 1 runs the Monte Carlo solve, otherwise the
+// (src/main.rs:336-353): 1 runs the Monte Carlo solve, otherwise the
 // finite-difference diffusion solve (nalgebra_method) for `solver` 0; the
 // reference implements no other solver.  --mc / --diffusion override it.
 // Monte Carlo defaults: the deck's histories/generations/skip, FP32

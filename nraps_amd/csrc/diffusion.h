@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <cstdint>
 #include <vector>
 
@@ -58,8 +59,10 @@ class DiffusionSet {
   void upload(DevArena &arena, hipStream_t s);
   // One launch for every problem of the set.
   void launch(hipStream_t s);
-  // Copy problem i's results into r after the stream has run the launch.
-  int download(int i, nraps_diffusion_result *r, hipStream_t s);
+  // Copy every problem's outputs back (one copy; synchronises s), then
+  // unpack problem i's results into r.
+  void fetch(hipStream_t s);
+  int download(int i, nraps_diffusion_result *r);
   const DiffProblem *dev_problem(int i) const { return d_probs_ + slot_[i]; }
   int size() const { return (int)host_.size(); }
 
@@ -88,6 +91,10 @@ class DiffusionSet {
   DiffProblem *d_probs_ = nullptr;
   std::vector<DiffProblem> h_probs_;
   std::vector<uint8_t> stage_;  // source of the asynchronous upload
+  uint8_t *d_base_ = nullptr;
+  size_t out_begin_ = 0, out_end_ = 0;           // device byte range of the outputs
+  std::vector<std::array<size_t, 3>> out_off_;   // per problem: flux, fs, scal offsets
+  std::vector<uint8_t> out_;                     // host copy of the outputs
 };
 
 // Fission-bank start of a Monte Carlo problem: N sources at the quantiles

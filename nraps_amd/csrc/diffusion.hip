@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -661,6 +662,11 @@ void DiffusionSet::upload(DevArena &arena, hipStream_t s) {
     size_t gflux, gF, flux, fs, scal;
   };
   std::vector<Off> off(np);
+  auto off_outputs = [](const std::vector<Off> &o) {
+    std::vector<std::array<size_t, 3>> v(o.size());
+    for (size_t i = 0; i < o.size(); i++) v[i] = {o[i].flux, o[i].fs, o[i].scal};
+    return v;
+  };
   const size_t probs = put(sizeof(DiffProblem) * np);
   for (int i = 0; i < np; i++) {
     const Host &h = host_[i];
@@ -683,15 +689,24 @@ void DiffusionSet::upload(DevArena &arena, hipStream_t s) {
   const size_t up_end = at;
   for (int i = 0; i < np; i++) {
     const Host &h = host_[i];
-    const size_t n = h.d.n, G = h.d.G, LT = (size_t)h.d.L * h.d.bs;
+    const size_t G = h.d.G, LT = (size_t)h.d.L * h.d.bs;
     Off &f = off[i];
     f.gflux = h.d.in_lds ? 0 : put(G * LT * 8);
     f.gF = h.d.in_lds ? 0 : put(LT * 8);
+  }
+  // outputs of every problem in one range: one copy back for the set
+  out_begin_ = at;
+  for (int i = 0; i < np; i++) {
+    const size_t n = host_[i].d.n, G = host_[i].d.G;
+    Off &f = off[i];
     f.flux = put(G * n * 8);
     f.fs = put(n * 8);
     f.scal = put(8 * 8);
   }
+  out_end_ = at;
+  out_off_ = off_outputs(off);
   uint8_t *base = arena.alloc<uint8_t>(at);
+  d_base_ = base;
   std::vector<uint8_t> stage(up_end, 0);
   h_probs_.resize(np);
   for (int i = 0; i < np; i++) {
@@ -784,19 +799,21 @@ void DiffusionSet::launch(hipStream_t s) {
   }
 }
 
-int DiffusionSet::download(int i, nraps_diffusion_result *r, hipStream_t s) {
+void DiffusionSet::fetch(hipStream_t s) {
+  out_.resize(out_end_ - out_begin_);
+  HIP_TRY(hipMemcpyAsync(out_.data(), d_base_ + out_begin_, out_.size(), hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+}
+
+int DiffusionSet::download(int i, nraps_diffusion_result *r) {
   const DiffProblem &d = h_probs_[i];
   const size_t n = d.n, G = d.G;
-  std::vector<double> flux(G * n), fs(n);
-  double scal[5];
-  HIP_TRY(hipMemcpyAsync(flux.data(), d.flux_out, G * n * 8, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(fs.data(), d.fs_out, n * 8, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipMemcpyAsync(scal, d.scal, sizeof scal, hipMemcpyDeviceToHost, s));
-  HIP_TRY(hipStreamSynchronize(s));
+  auto at = [&](size_t off) { return reinterpret_cast<const double *>(&out_[off - out_begin_]); };
+  const double *flux = at(out_off_[i][0]), *fs = at(out_off_[i][1]), *scal = at(out_off_[i][2]);
   const int status = (int)scal[4];
   if (!r) return status;
-  if (r->flux) std::memcpy(r->flux, flux.data(), G * n * 8);
-  if (r->fission_source) std::memcpy(r->fission_source, fs.data(), n * 8);
+  if (r->flux) std::memcpy(r->flux, flux, G * n * 8);
+  if (r->fission_source) std::memcpy(r->fission_source, fs, n * 8);
   if (r->assembly_average) {
     // average_assembly, src/discrete.rs:151-167
     std::fill(r->assembly_average, r->assembly_average + G * n, 0.0);
@@ -902,9 +919,10 @@ int diffusion_batch(int n, const nraps_problem *const *pbs, const nraps_diffusio
       HIP_TRY(hipStreamSynchronize(s));
       float ms_dev = 0.f;
       HIP_TRY(hipEventElapsedTime(&ms_dev, sg.e0, sg.e1));
+      set.fetch(s);
       for (int i = 0; i < n; i++) {
         nraps_diffusion_result *r = results ? results[i] : nullptr;
-        const int st = set.download(i, r, s);
+        const int st = set.download(i, r);
         if (st != NRAPS_OK && worst == NRAPS_OK) worst = st;
         if (r) r->ms_device = ms_dev;
       }

@@ -45,7 +45,20 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0, help="histories per wave claim (0: automatic)")
     ap.add_argument("--bank", action="store_true", help="fission-bank source (skip >= 2)")
+    ap.add_argument("--diffusion", type=int, default=0,
+                    help="instead: diffusion solve of this many problems of the config "
+                         "(the uneven batch for config 3/4), --generations times")
     a = ap.parse_args()
+    if a.diffusion:
+        probs = [nr.generate(a.config, a.index + i)[0] for i in range(a.diffusion)] \
+            if a.config >= 3 else [nr.generate(a.config, a.index)[0]] * a.diffusion
+        for _ in range(max(a.generations, 1)):
+            rs = nr.diffusion_solve_batch(probs)
+        its = [r.iterations for r in rs]
+        print(f"config {a.config}: diffusion of {len(probs)} problem(s), iterations "
+              f"{min(its)}..{max(its)}, device {rs[0].ms_device:.3f} ms, k[0] {rs[0].k:.8f}",
+              flush=True)
+        return
     p, o = nr.generate(a.config, a.index)
     if a.bank:
         o.source = _abi.SOURCE_FISSION_BANK
