@@ -261,11 +261,17 @@ def test_device_diffusion_matches_oracle(gpu, oracle, name, make):
 
 
 @pytest.mark.gpu
-def test_device_diffusion_global_memory_form(gpu, oracle):
-    """A problem too large for LDS runs with flux in global memory."""
-    p = random_slab(9, G=8, regions=41, cells=60, bl=0.0, br=1.0)
+@pytest.mark.parametrize("G,regions,cells", [(8, 41, 60), (10, 61, 70)])
+def test_device_diffusion_global_memory_form(gpu, oracle, monkeypatch, G, regions, cells):
+    """Problems too large for LDS run with the flux in global memory: a
+    2460-cell 8-group slab on the LDS/global kernel (forced) and a
+    4270-cell 10-group slab, which takes that kernel by itself (more than 8
+    groups, 17 cells per thread)."""
+    p = random_slab(9, G=G, regions=regions, cells=cells, bl=0.0, br=1.0)
     assert (p.n_groups + 1) * p.n_cells * 8 > 160 * 1024
-    print(f"diffusion parity: global form, {p.n_cells} cells", flush=True)
+    if G <= 8:
+        monkeypatch.setenv("NRAPS_DIFFUSION_GENERIC", "1")
+    print(f"diffusion parity: global form, {p.n_cells} cells, {G} groups", flush=True)
     _compare(nr.diffusion_solve(p, check=False), oracle.diffusion(p), p)
 
 
@@ -335,3 +341,26 @@ def test_diffusion_started_bank_bit_exact(gpu, oracle, method):
     # (the event path's difference-array tally leaves ~1e-15-relative dust in
     # cells whose oracle flux is exactly zero)
     np.testing.assert_allclose(got.flux, want.flux, rtol=1e-10, atol=1e-12 * np.abs(want.flux).max())
+
+
+@pytest.mark.gpu
+def test_diffusion_start_in_batch_and_server(gpu):
+    """Diffusion-started FP32 requests give the same k, bit for bit, solved
+    alone, in one batch and through the continuous-batching server."""
+    probs = []
+    for i in range(4):
+        p, o = nr.generate(3, i)
+        o.histories, o.generations, o.skip = min(int(o.histories), 15000), 8, 2
+        o.bank_init = _abi.BANK_INIT_DIFFUSION
+        probs.append((p, o))
+    print("diffusion start: solo, batch, server", flush=True)
+    solo = [nr.solve(p, o) for p, o in probs]
+    batch = nr.solve_batch([p for p, _ in probs], [o for _, o in probs])
+    with nr.Server(precision=_abi.FP32, budget_bytes=8 * 10**9, max_slots=2) as srv:
+        tickets = [srv.submit(p, o) for p, o in probs]
+        served = [srv.wait(t, timeout_ms=120000) for t in tickets]
+    for s, b, v in zip(solo, batch, served):
+        assert s.status_code == 0 and v.status_code == 0
+        assert s.sites[0] > 0  # generation 0 banks sites
+        assert np.array_equal(s.k.view(np.uint64), b.k.view(np.uint64))
+        assert np.array_equal(s.k.view(np.uint64), v.k.view(np.uint64))
