@@ -63,6 +63,32 @@ __device__ inline uint64_t uniform_u64(uint64_t v) {
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
   return ((uint64_t)hi << 32) | lo;
 }
+// Lane 0's copy of a wave-uniform value.  Fields read from the problem
+// descriptor arrive through vector loads (the kernel writes global memory,
+// so the compiler cannot use scalar loads for them); passing them through
+// readfirstlane moves them to scalar registers for the rest of the visit.
+__device__ inline int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline int64_t uni(int64_t v) { return (int64_t)uniform_u64((uint64_t)v); }
+__device__ inline uint64_t uni(uint64_t v) { return uniform_u64(v); }
+__device__ inline float uni(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ inline double uni(double v) {
+  return __longlong_as_double((long long)uniform_u64((uint64_t)__double_as_longlong(v)));
+}
+template <class T>
+__device__ inline T *uni(T *ptr) {  // global-memory pointers only (not LDS)
+  return reinterpret_cast<T *>(uniform_u64(reinterpret_cast<uint64_t>(ptr)));
+}
+// uni() where U holds: a kernel whose vector registers are scarce moves the
+// descriptor to scalar registers; one whose scalar registers are the tighter
+// budget keeps it in vector registers.
+template <bool U, class T>
+__device__ inline T uni_if(T v) {
+  if constexpr (U) return uni(v);
+  else return v;
+}
+
 __device__ inline int rank_below(uint64_t mask) {
   return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
                                         __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
@@ -103,23 +129,27 @@ struct Tabs {
   const int32_t *fuel;
   const double *nsfd, *reglo, *kreg;
   const uint16_t *bucket;
+  // The offsets are wave-uniform: with U, uni() keeps them in scalar
+  // registers (the base keeps its address space, so LDS tables stay LDS
+  // accesses).
+  template <bool U = true>
   __device__ void bind(const uint8_t *base, const uint8_t *global_base, const TabLayout &L) {
-    edges = (const Real *)(base + L.edges);
-    mat8 = base + L.mat8;
-    reg16 = (const uint16_t *)(base + L.reg16);
-    fuel = (const int32_t *)(global_base + L.fuel);
-    dx = (const Real *)(global_base + L.dx);
-    sigw = (const Real *)(base + L.sigw);
-    invtr = (const Real *)(base + L.invtr);
-    absr = (const Real *)(base + L.absr);
-    invs = (const Real *)(base + L.invs);
-    nsf = (const Real *)(base + L.nsf);
-    chicdf = (const Real *)(base + L.chicdf);
-    cumrow = (const Real *)(base + L.cumrow);
-    nsfd = (const double *)(base + L.nsfd);
-    reglo = (const double *)(base + L.reglo);
-    kreg = (const double *)(base + L.kreg);
-    bucket = (const uint16_t *)(base + L.bucket);
+    edges = (const Real *)(base + uni_if<U>(L.edges));
+    mat8 = base + uni_if<U>(L.mat8);
+    reg16 = (const uint16_t *)(base + uni_if<U>(L.reg16));
+    fuel = (const int32_t *)(global_base + uni_if<U>(L.fuel));
+    dx = (const Real *)(global_base + uni_if<U>(L.dx));
+    sigw = (const Real *)(base + uni_if<U>(L.sigw));
+    invtr = (const Real *)(base + uni_if<U>(L.invtr));
+    absr = (const Real *)(base + uni_if<U>(L.absr));
+    invs = (const Real *)(base + uni_if<U>(L.invs));
+    nsf = (const Real *)(base + uni_if<U>(L.nsf));
+    chicdf = (const Real *)(base + uni_if<U>(L.chicdf));
+    cumrow = (const Real *)(base + uni_if<U>(L.cumrow));
+    nsfd = (const double *)(base + uni_if<U>(L.nsfd));
+    reglo = (const double *)(base + uni_if<U>(L.reglo));
+    kreg = (const double *)(base + uni_if<U>(L.kreg));
+    bucket = (const uint16_t *)(base + uni_if<U>(L.bucket));
   }
   __device__ int mat(int c) const { return (int)mat8[c]; }
   __device__ int reg(int c) const { return (int)reg16[c]; }
@@ -151,47 +181,49 @@ struct Ctx {
   int64_t trace_n;
   int trace_ev, trace_cycle;
 
+  // U: descriptor fields through uni(); UT: the table offsets.
+  template <bool U = true, bool UT = U>
   __device__ void load(const DevProblem &P, const CycleState &cs, const uint8_t *tab) {
-    T.bind(tab, P.tab, P.lay);
-    nc = P.nc;
-    G = P.G;
-    M = P.M;
-    R = P.R;
-    nf = P.nf;
-    nb = P.nb;
-    xs_every = P.xs_every;
-    fission = P.fission;
-    from_bank = cs.from_bank;
-    make_sites = cs.make_sites;
-    cycle = cs.cycle;
-    max_events = P.max_events;
-    N = P.N;
-    seed = P.seed;
-    problem_id = P.problem_id;
-    maj = (Real)P.maj;
-    inv_maj = (Real)P.inv_maj;
-    XL = (Real)P.XL;
-    XR = (Real)P.XR;
-    bl = (Real)P.boundl;
-    br = (Real)P.boundr;
-    inv_k = (Real)cs.inv_k;
-    inv_h = (Real)P.inv_h;
-    kacc = P.kacc;
-    counts = P.counts;
-    src_x = (const Real *)P.src_x;
-    src_cell = P.src_cell;
-    site_x = (Real *)P.site_x;
-    site_cell = P.site_cell;
-    site_key = P.site_key;
-    site_count = &P.cs->site_count;
-    site_cap = P.site_cap;
-    trace = P.trace;
-    trace_len = P.trace_len;
-    draws = P.draws;
-    trace_kacc = P.trace_kacc;
-    trace_n = P.trace_n;
-    trace_ev = P.trace_ev;
-    trace_cycle = P.trace_cycle;
+    T.template bind<UT>(tab, uni_if<UT>(P.tab), P.lay);
+    nc = uni_if<U>(P.nc);
+    G = uni_if<U>(P.G);
+    M = uni_if<U>(P.M);
+    R = uni_if<U>(P.R);
+    nf = uni_if<U>(P.nf);
+    nb = uni_if<U>(P.nb);
+    xs_every = uni_if<U>(P.xs_every);
+    fission = uni_if<U>(P.fission);
+    from_bank = uni_if<U>(cs.from_bank);
+    make_sites = uni_if<U>(cs.make_sites);
+    cycle = uni_if<U>(cs.cycle);
+    max_events = uni_if<U>(P.max_events);
+    N = uni_if<U>(P.N);
+    seed = uni_if<U>(P.seed);
+    problem_id = uni_if<U>(P.problem_id);
+    maj = uni_if<U>((Real)P.maj);
+    inv_maj = uni_if<U>((Real)P.inv_maj);
+    XL = uni_if<U>((Real)P.XL);
+    XR = uni_if<U>((Real)P.XR);
+    bl = uni_if<U>((Real)P.boundl);
+    br = uni_if<U>((Real)P.boundr);
+    inv_k = uni_if<U>((Real)cs.inv_k);
+    inv_h = uni_if<U>((Real)P.inv_h);
+    kacc = uni_if<U>(P.kacc);
+    counts = uni_if<U>(P.counts);
+    src_x = uni_if<U>((const Real *)P.src_x);
+    src_cell = uni_if<U>(P.src_cell);
+    site_x = uni_if<U>((Real *)P.site_x);
+    site_cell = uni_if<U>(P.site_cell);
+    site_key = uni_if<U>(P.site_key);
+    site_count = &uni_if<U>(P.cs)->site_count;
+    site_cap = uni_if<U>(P.site_cap);
+    trace = uni_if<U>(P.trace);
+    trace_len = uni_if<U>(P.trace_len);
+    draws = uni_if<U>(P.draws);
+    trace_kacc = uni_if<U>(P.trace_kacc);
+    trace_n = uni_if<U>(P.trace_n);
+    trace_ev = uni_if<U>(P.trace_ev);
+    trace_cycle = uni_if<U>(P.trace_cycle);
   }
 };
 
@@ -733,16 +765,28 @@ struct WaveStats {
 //   FP32, 256-thread blocks, no fission: 5 waves (96 VGPRs, spills only in
 //     the refill and boundary paths), 5.6% faster than 4 waves with 127
 //     VGPRs and no spills; 6 waves is slower;
-//   FP32, 256-thread blocks, fission sites: 4 waves (128 VGPRs), 4.5% faster
-//     than the allocator's 156 VGPRs at 3 waves;
+//   FP32, 256-thread blocks, fission sites: 4 waves (94 VGPRs with the
+//     descriptor in scalar registers), 0.6% faster than 5 waves; with the
+//     descriptor in vector registers it needed 128 VGPRs and spilled;
 //   1024-thread blocks run 4 waves per SIMD whatever the count; the FP64
 //     and trace variants keep the allocator's choice (at least 3 waves).
 #ifndef NRAPS_WC_WAVES_PER_EU
 #define NRAPS_WC_WAVES_PER_EU 3
 #endif
+#ifndef NRAPS_WC_HOT_UNI_TABLES
+#define NRAPS_WC_HOT_UNI_TABLES 0
+#endif
+#ifndef NRAPS_WC_FP32_WAVES
+#define NRAPS_WC_FP32_WAVES 5
+#endif
+#ifndef NRAPS_WC_FP32_FISSION_WAVES
+#define NRAPS_WC_FP32_FISSION_WAVES 4
+#endif
 template <typename Real, bool TRACE, bool FISSION, int BS>
 constexpr int wc_waves_per_eu() {
-  return (sizeof(Real) == 4 && !TRACE && BS == 256) ? (FISSION ? 4 : 5) : NRAPS_WC_WAVES_PER_EU;
+  return (sizeof(Real) == 4 && !TRACE && BS == 256)
+             ? (FISSION ? NRAPS_WC_FP32_FISSION_WAVES : NRAPS_WC_FP32_WAVES)
+             : NRAPS_WC_WAVES_PER_EU;
 }
 
 // Persistent event kernel.  A block visits the problems of the batch in
@@ -793,7 +837,12 @@ void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim
     __syncthreads();
 
     Ctx<Real> C;
-    C.load(P, cs, smem);
+    // descriptor in scalar registers for every variant but the 256-thread
+    // FP32 one without fission sites: that one measured 0.8% faster with it
+    // in vector registers (96 VGPRs, a few spills) than in scalar registers
+    // (83 VGPRs), while the fission variant gained 6% from scalar registers
+    constexpr bool kHot = sizeof(Real) == 4 && !FISSION && !TRACE && BS == 256;
+    C.template load<!kHot, !kHot || NRAPS_WC_HOT_UNI_TABLES>(P, cs, smem);
     Acc *direct = LDS_TALLY ? (Acc *)ltally : (Acc *)P.tally_glob;
     Acc *diff = direct + C.G * C.nc;
     unsigned long long *queue = (unsigned long long *)&P.cs->queue_head;
