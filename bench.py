@@ -176,6 +176,15 @@ def main():
                                          f"{args.cpu_generations} generations, {wc:.1f} s"}
         out["speedup_vs_cpu"] = value / cpu_v
         log(f"CPU oracle: {cpu_v:.4g} histories/s on {threads} threads")
+        # the same sample on one more thread (all of the 16-core share: the
+        # reference uses available_parallelism() - 1, src/mc_code.rs:302)
+        t = time.perf_counter()
+        oracle_lib.solve(prob, oc, threads=threads + 1)
+        wa = time.perf_counter() - t
+        out["cpu_baseline"]["value_one_more_thread"] = args.cpu_histories * args.cpu_generations / wa
+        out["cpu_baseline"]["cores_one_more_thread"] = threads + 1
+        log(f"CPU oracle: {out['cpu_baseline']['value_one_more_thread']:.4g} histories/s "
+            f"on {threads + 1} threads")
         if "diffusion" in out:
             # the FP64 CPU oracle of the diffusion solver (one core, O(n) solves)
             t = time.perf_counter()

@@ -100,7 +100,7 @@ class DiffusionSet {
 // Fission-bank start of a Monte Carlo problem: N sources at the quantiles
 // (p + 1/2)/N of problem dp's fission source, generation-0 state of cs
 // (1/k_diffusion for site production).  cdf: n doubles of scratch.
-void launch_diffusion_bank(bool fp32, const DiffProblem *dp, int n_cells, int64_t N, void *src_x,
+void launch_diffusion_bank(bool fp32, const DiffProblem *dp, int64_t N, void *src_x,
                            int32_t *src_cell, CycleState *cs, double *cdf, hipStream_t s);
 
 }  // namespace nraps

@@ -836,9 +836,8 @@ int DiffusionSet::download(int i, nraps_diffusion_result *r) {
   return status;
 }
 
-void launch_diffusion_bank(bool fp32, const DiffProblem *dp, int n_cells, int64_t N, void *src_x,
+void launch_diffusion_bank(bool fp32, const DiffProblem *dp, int64_t N, void *src_x,
                            int32_t *src_cell, CycleState *cs, double *cdf, hipStream_t s) {
-  (void)n_cells;
   hipLaunchKernelGGL(k_dbank_cdf, dim3(1), dim3(64), 0, s, dp, cdf, cs);
   HIP_TRY(hipGetLastError());
   const int bs = 256;

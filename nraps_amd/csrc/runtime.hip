@@ -253,7 +253,7 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
     diff->upload(arena, s);
     diff->launch(s);
     double *cdf = arena.alloc<double>(nc);
-    launch_diffusion_bank(fp32, diff->dev_problem(0), nc, N, d.src_x, d.src_cell, d_cs, cdf, s);
+    launch_diffusion_bank(fp32, diff->dev_problem(0), N, d.src_x, d.src_cell, d_cs, cdf, s);
   }
 
   DevFinal &f = fin;

@@ -247,6 +247,12 @@ void nraps_server::loop() {
           std::lock_guard<std::mutex> g(m);
           resident.push_back(r);
           dirty = true;
+        } catch (const std::invalid_argument &e) {
+          // e.g. a diffusion-started bank whose diffusion matrix is singular
+          set_error(e.what());
+          NRAPS_HIP_TRY(hipStreamSynchronize(s));
+          r->dev.reset();
+          retire(r, NRAPS_E_INVALID);
         } catch (const std::exception &e) {
           set_error(e.what());
           NRAPS_HIP_TRY(hipStreamSynchronize(s));
