@@ -50,33 +50,81 @@ struct Aff {
 };
 __device__ __forceinline__ Aff compose(Aff p, Aff q) { return Aff{q.a * p.a, q.a * p.b + q.b}; }
 
+// DPP controls (GFX9 encoding): shift right within a row of 16 lanes, lane
+// 15 of a row to the next row, lane 31 to rows 2 and 3, whole wave right by 1.
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114, kDppRowShr8 = 0x118;
+constexpr int kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143, kDppWaveShr1 = 0x138;
+
+// A double moved across lanes by DPP; lanes without a source (or outside
+// ROW_MASK) get `old`.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v, double old) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(v), CTRL, ROW_MASK,
+                                             0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(v), CTRL, ROW_MASK,
+                                             0xf, false);
+  return __hiloint2double(hi, lo);
+}
+// The map of the lane CTRL names, or the identity where there is none.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ Aff dpp_take(Aff v) {
+  return Aff{dpp_f64<CTRL, ROW_MASK>(v.a, 1.0), dpp_f64<CTRL, ROW_MASK>(v.b, 0.0)};
+}
+__device__ __forceinline__ Aff readlane(Aff v, int l) {
+  auto rl = [&](double x) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), l),
+                            __builtin_amdgcn_readlane(__double2loint(x), l));
+  };
+  return Aff{rl(v.a), rl(v.b)};
+}
+__device__ __forceinline__ Aff lane_from(Aff v, int src) {
+  return Aff{__shfl(v.a, src), __shfl(v.b, src)};
+}
+
+// Inclusive scan over lanes 0..63 in lane order: rows of 16 by shifts of
+// 1, 2, 4, 8, then the row totals by two broadcasts (all DPP moves).
+__device__ __forceinline__ Aff wave_inscan(Aff v) {
+  v = compose(dpp_take<kDppRowShr1>(v), v);
+  v = compose(dpp_take<kDppRowShr2>(v), v);
+  v = compose(dpp_take<kDppRowShr4>(v), v);
+  v = compose(dpp_take<kDppRowShr8>(v), v);
+  v = compose(dpp_take<kDppRowBcast15, 0xa>(v), v);
+  v = compose(dpp_take<kDppRowBcast31, 0xc>(v), v);
+  return v;
+}
+
 // Exclusive scan of one affine map per thread of a BS-thread workgroup, in
 // thread order (FWD) or in reverse thread order; wtot: 2*BS/64 doubles of LDS.
+// The reverse order reverses the lanes (one shuffle each way) and scans
+// forward.  The wave totals are combined by a scan over the first BS/64
+// lanes of every wave.
 template <bool FWD, bool SYNC_AFTER, int BS>
 __device__ __forceinline__ Aff block_exscan(Aff v, double *wtot) {
   constexpr int kW = BS / 64;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  Aff inc = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const double pa = FWD ? __shfl_up(inc.a, d) : __shfl_down(inc.a, d);
-    const double pb = FWD ? __shfl_up(inc.b, d) : __shfl_down(inc.b, d);
-    if (FWD ? lane >= d : lane + d < 64) inc = compose(Aff{pa, pb}, inc);
-  }
-  const double ea = FWD ? __shfl_up(inc.a, 1) : __shfl_down(inc.a, 1);
-  const double eb = FWD ? __shfl_up(inc.b, 1) : __shfl_down(inc.b, 1);
-  const Aff ex = (FWD ? lane > 0 : lane < 63) ? Aff{ea, eb} : Aff{1.0, 0.0};
-  if (lane == (FWD ? 63 : 0)) {
-    wtot[2 * w] = inc.a;
-    wtot[2 * w + 1] = inc.b;
+  const Aff in = FWD ? v : lane_from(v, 63 - lane);
+  const Aff inc = wave_inscan(in);
+  const Aff ex_s = dpp_take<kDppWaveShr1>(inc);  // exclusive, in scan order
+  const Aff ex = FWD ? ex_s : lane_from(ex_s, 63 - lane);
+  const Aff tot = readlane(inc, 63);
+  if (lane == 0) {
+    wtot[2 * w] = tot.a;
+    wtot[2 * w + 1] = tot.b;
   }
   __syncthreads();
-  Aff pre{1.0, 0.0};
-  if (FWD) {
-    for (int u = 0; u < w; u++) pre = compose(pre, Aff{wtot[2 * u], wtot[2 * u + 1]});
-  } else {
-    for (int u = kW - 1; u > w; u--) pre = compose(pre, Aff{wtot[2 * u], wtot[2 * u + 1]});
+  // scan of the wave totals in scan order: lane r holds wave r (FWD) or
+  // wave kW-1-r
+  Aff t{1.0, 0.0};
+  if (lane < kW) {
+    const int u = FWD ? lane : kW - 1 - lane;
+    t = Aff{wtot[2 * u], wtot[2 * u + 1]};
   }
+  t = compose(dpp_take<kDppRowShr1>(t), t);
+  if (kW > 2) t = compose(dpp_take<kDppRowShr2>(t), t);
+  if (kW > 4) t = compose(dpp_take<kDppRowShr4>(t), t);
+  if (kW > 8) t = compose(dpp_take<kDppRowShr8>(t), t);
+  const int r = FWD ? w : kW - 1 - w;  // this wave's place in scan order
+  const Aff pre = r > 0 ? readlane(t, r - 1) : Aff{1.0, 0.0};
   if (SYNC_AFTER) __syncthreads();  // wtot is written again by the next scan
   return compose(pre, ex);
 }
