@@ -9,8 +9,8 @@
 // factors it once (Gaussian elimination without pivoting, the Thomas
 // algorithm), and the device solves each group in O(n): the forward and back
 // substitutions are first-order linear recurrences, which one workgroup of 256
-// threads evaluates as a parallel scan of affine maps (per-thread runs of L
-// cells, wave shuffles, one LDS step across the 4 waves).  The whole power
+// or 1024 threads evaluates as a parallel scan of affine maps (per-thread runs
+// of L cells, DPP moves inside a wave, one LDS step across the waves).  The whole power
 // iteration, convergence test included, runs in one launch with no host round
 // trip; a batch puts one problem on each workgroup.
 //
