@@ -155,7 +155,28 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
   B.stats = put(sizeof(unsigned long long) * kNumStats);
   B.kpart = put((size_t)P.G * 8);
   B.end = at;
-  uint8_t *base = arena.alloc<uint8_t>(B.end);
+  // the large per-history and bank arrays follow in the same allocation:
+  // one pool allocation (and one free) per problem
+  const int64_t site_cap = fission ? site_capacity(N) : 0;
+  const size_t o_kacc = put((size_t)N * 8);
+  const size_t o_tally = put((size_t)2 * P.G * P.nc * 8);
+  size_t o_counts = 0, o_offsets = 0, o_sx = 0, o_scell = 0, o_skey = 0, o_ox = 0, o_ocell = 0,
+         o_srcx = 0, o_srccell = 0;
+  if (fission) {
+    o_counts = put((size_t)N * 4);
+    o_offsets = put((size_t)N * 8);
+    o_sx = put((size_t)site_cap * rs);
+    o_scell = put((size_t)site_cap * 4);
+    o_skey = put((size_t)site_cap * 8);
+    o_ox = put((size_t)site_cap * rs);
+    o_ocell = put((size_t)site_cap * 4);
+    o_srcx = put((size_t)N * rs);
+    o_srccell = put((size_t)N * 4);
+  }
+  const size_t o_tree_a = put((size_t)(ceil_div(N, kTreeWidth) + 1) * 8);
+  const size_t o_tree_b = put((size_t)(ceil_div(N, 65536) + 1) * 8);
+  const size_t total = at;
+  uint8_t *base = arena.alloc<uint8_t>(total);
   d_base = base;
 
   std::memset(&cs0, 0, sizeof cs0);
@@ -209,22 +230,22 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
   d.cs = d_cs;
   d.k = reinterpret_cast<double *>(base + B.k);
   d.stats = reinterpret_cast<unsigned long long *>(base + B.stats);
-  d.kacc = arena.alloc<double>(N);
-  d.tally_glob = arena.alloc<double>(d.tally_bins);
+  d.kacc = reinterpret_cast<double *>(base + o_kacc);
+  d.tally_glob = reinterpret_cast<double *>(base + o_tally);
   if (fission) {
-    d.counts = arena.alloc<int32_t>(N);
-    d.offsets = arena.alloc<int64_t>(N);
-    d.site_cap = site_capacity(N);
-    d.site_x = arena.raw((size_t)d.site_cap * rs);
-    d.site_cell = arena.alloc<int32_t>(d.site_cap);
-    d.site_key = arena.alloc<uint64_t>(d.site_cap);
-    d.ord_x = arena.raw((size_t)d.site_cap * rs);
-    d.ord_cell = arena.alloc<int32_t>(d.site_cap);
-    d.src_x = arena.raw((size_t)N * rs);
-    d.src_cell = arena.alloc<int32_t>(N);
+    d.counts = reinterpret_cast<int32_t *>(base + o_counts);
+    d.offsets = reinterpret_cast<int64_t *>(base + o_offsets);
+    d.site_cap = site_cap;
+    d.site_x = base + o_sx;
+    d.site_cell = reinterpret_cast<int32_t *>(base + o_scell);
+    d.site_key = reinterpret_cast<uint64_t *>(base + o_skey);
+    d.ord_x = base + o_ox;
+    d.ord_cell = reinterpret_cast<int32_t *>(base + o_ocell);
+    d.src_x = base + o_srcx;
+    d.src_cell = reinterpret_cast<int32_t *>(base + o_srccell);
   }
-  d.tree_a = arena.alloc<double>(ceil_div(N, kTreeWidth) + 1);
-  d.tree_b = arena.alloc<double>(ceil_div(N, 65536) + 1);
+  d.tree_a = reinterpret_cast<double *>(base + o_tree_a);
+  d.tree_b = reinterpret_cast<double *>(base + o_tree_b);
   trace_n = 0;
   if (want_trace && o.trace_histories > 0 && r && r->trace && r->trace_len && r->draws && r->kacc) {
     trace_n = o.trace_histories;

@@ -133,12 +133,21 @@ def run_batch(n, *, use_graph=1, precision=_abi.FP32, generations=None):
             o.generations = generations
         probs.append(p)
         opts.append(o)
+    # warm-up: a short batch of the first problems loads the kernels and
+    # grows the device memory pool, as bench.py's warm-up steps do
+    wo = []
+    for o in opts[:8]:
+        w = type(o).from_buffer_copy(o)
+        w.generations, w.skip = 3, 2
+        wo.append(w)
+    nr.solve_batch(probs[:8], wo)
     t = time.perf_counter()
     res = nr.solve_batch(probs, opts)
     wall = time.perf_counter() - t
     hist = sum(r.histories_run for r in res)
     return {"problems": n, "histories": hist, "wall_s": wall, "histories_per_s": hist / wall,
             "generations": int(opts[0].generations),
+            "library_s": res[0].ms_total * 1e-3, "device_loop_s": res[0].ms_device * 1e-3,
             "k_min": min(r.k_mean for r in res), "k_max": max(r.k_mean for r in res)}
 
 
