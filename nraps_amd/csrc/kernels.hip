@@ -51,7 +51,10 @@ template <>
 struct Num<float> {
   __device__ static float u01(uint32_t u) { return nraps_u01_f32(u); }
   __device__ static float flight(uint32_t u) { return nraps_uflight_f32(u); }
-  __device__ static float nlog(float v) { return __logf(v); }
+  // v_log_f32 (log2) times ln 2: flight arguments are >= 2^-32, never
+  // denormal, so no range reduction is needed (1.6% of the event kernel
+  // against __logf)
+  __device__ static float nlog(float v) { return __builtin_amdgcn_logf(v) * 0.69314718f; }
   __device__ static float rcp(float v) { return __builtin_amdgcn_rcpf(v); }
 };
 
