@@ -6,13 +6,14 @@ per region, reflective left / vacuum right), 10^7 particles per generation,
 20 inactive + 100 active generations, seeded synthetic cross sections.
 
 A step is one full k-eigenvalue solve of that workload (120 generations).
-`value` is the FP32 event-based (Woodcock) histories/s over the K timed
-steps; the plain FP64 port (one thread per history, surface tracking) runs
+`value` is the FP32 event-kernel histories/s over the K timed steps
+(region-by-region surface tracking; `--method woodcock` selects Woodcock
+delta tracking); the plain FP64 port (one thread per history, surface tracking) runs
 the same workload once beside it, and the FP64 CPU oracle is timed on a
 bounded sample on the host cores.  The serving form runs a 10 s Poisson
 request stream (50 req/s) beside them for the p50/p99 solve latency.
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--method region|woodcock]
 """
 import argparse
 import json
@@ -26,11 +27,13 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 B_STATE_FP32, B_STATE_FP64 = 24, 32
-# HBM bytes per event-kernel launch on this workload, from two rocprofv3 counter
-# passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors calibrated on a
-# 1 GiB device copy): 15514.7 KiB x 2 read + 159322.7 KiB written, per launch
-# of 10^7 histories (profiles/k_woodcock_9region_4group_hbm_traffic.md).
-TRAFFIC_BYTES_PER_LAUNCH = (15514.7 * 2 + 159322.7) * 1024
+# HBM bytes per event-kernel launch of 10^7 histories on this workload, from
+# two rocprofv3 counter passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000,
+# factors calibrated on a 1 GiB device copy): region tracking 95.8 KiB x 2
+# read + 106016.3 KiB written (profiles/k_event_region_9region_4group_hbm_traffic.md),
+# Woodcock 15514.7 KiB x 2 + 159322.7 KiB (profiles/k_woodcock_9region_4group_hbm_traffic.md).
+TRAFFIC_BYTES_PER_LAUNCH = {"region": (95.78125 * 2 + 106016.34375) * 1024,
+                            "woodcock": (15514.7 * 2 + 159322.7) * 1024}
 
 
 def main():
@@ -46,6 +49,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-seconds", type=float, default=10.0)
     ap.add_argument("--no-diffusion", action="store_true", help="skip the diffusion leg")
+    ap.add_argument("--method", choices=["region", "woodcock"], default="region",
+                    help="tracking of the FP32 event kernel")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -62,7 +67,8 @@ def main():
     with open(os.path.join(ROOT, "tests", "golden", "workloads.json")) as f:
         fbar = json.load(f)["workloads"]["slab9_g4_1e7"]["mean_flights"]
 
-    prob, o = nr.generate(1)           # 10^7 x (20 + 100), FP32 event Woodcock
+    prob, o = nr.generate(1)           # 10^7 x (20 + 100), FP32 event kernel
+    o.method = _abi.EVENT_REGION if args.method == "region" else _abi.EVENT_WOODCOCK
     o.use_graph = 0                    # per-generation HIP events around the transport kernel
     N, gens = int(o.histories), int(o.generations)
 
@@ -94,8 +100,10 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "9-region 4-group slab: 1e7 particles/generation, "
-                               "20 inactive + 100 active generations, FP32 event-based "
-                               "Woodcock tracking, FP64 k", "histories_per_generation": N,
+                               "20 inactive + 100 active generations, FP32 event kernel "
+                               + ("with region-by-region surface tracking" if args.method == "region"
+                                  else "with Woodcock delta tracking") + ", FP64 k",
+                   "histories_per_generation": N,
                    "generations": gens, "cells": prob.n_cells, "groups": prob.n_groups},
         "k_fp32": k32, "k_fp32_sigma": s32,
         # the solve's own clocks: generation loop on the device (HIP events)
@@ -108,8 +116,9 @@ def main():
     avg_ms = tr_ms / max(tr_launch, 1)
     achieved = b_hist * N / (avg_ms * 1e-3) / 1e9
     out["roofline"] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BYTES_PER_LAUNCH,
-                       "kernel": "k_woodcock<float,lds>", "avg_launch_ms": avg_ms,
+                       "frac": achieved / HBM_PEAK_GBS,
+                       "traffic": TRAFFIC_BYTES_PER_LAUNCH[args.method],
+                       "kernel": f"k_event<float,lds,{args.method}>", "avg_launch_ms": avg_ms,
                        "bytes_per_history": b_hist, "mean_flights": fbar}
 
     if not args.no_fp64:

@@ -61,6 +61,11 @@ extern "C" {
 #define NRAPS_HISTORY_SURFACE 0 /* plain port: one thread per history, cell-by-cell
                                    surface tracking, src/mc_code.rs:134-213 */
 #define NRAPS_EVENT_WOODCOCK 1  /* event pipeline with Woodcock delta tracking */
+#define NRAPS_EVENT_REGION 2    /* event pipeline with the reference's surface tracking
+                                   taken region by region: a flight is sampled at
+                                   birth, after a scatter and on entry to another
+                                   material region (src/mc_code.rs:142-149), cells
+                                   are crossed without events */
 
 #define NRAPS_SOURCE_UNIFORM_FUEL 0 /* reference source: uniform fuel cell each
                                        generation, src/mc_code.rs:39-53 */
@@ -106,7 +111,7 @@ typedef struct nraps_run_opts {
   uint64_t seed;      /* PCG32 seed (the reference seeds from the clock) */
   uint64_t problem_id; /* high bits of every particle stream id (< 2^19) */
   int32_t precision;  /* NRAPS_FP64 / NRAPS_FP32 */
-  int32_t method;     /* NRAPS_HISTORY_SURFACE / NRAPS_EVENT_WOODCOCK */
+  int32_t method;     /* NRAPS_HISTORY_SURFACE / NRAPS_EVENT_WOODCOCK / NRAPS_EVENT_REGION */
   int32_t source;     /* NRAPS_SOURCE_* */
   int32_t xs_refresh; /* NRAPS_XS_* */
   double k_initial;   /* the k_new argument of monte_carlo (reference: 1.0) */
@@ -317,7 +322,9 @@ typedef struct nraps_server_opts {
                                SEM(k) / |mean k|, is <= this (50 pcm = 5e-4;
                                0 => off) */
   int32_t use_graph;
-  int32_t reserved[7];
+  int32_t method;           /* event method of every request: 0 or NRAPS_EVENT_REGION
+                               (region surface tracking), NRAPS_EVENT_WOODCOCK */
+  int32_t reserved[6];
 } nraps_server_opts;
 
 int nraps_server_create(const nraps_server_opts *opts, nraps_server **out);

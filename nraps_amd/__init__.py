@@ -196,7 +196,8 @@ class Server:
     """Continuous-batching request loop (nraps_server_*)."""
 
     def __init__(self, *, precision=_abi.FP32, budget_bytes=200 * 10**9, max_slots=256,
-                 min_active=10, sigma_k_target=0.0, use_graph=0, device=0):
+                 min_active=10, sigma_k_target=0.0, use_graph=0, device=0,
+                 method=_abi.EVENT_REGION):
         o = _abi.ServerOpts()
         o.device = device
         o.precision = precision
@@ -205,6 +206,7 @@ class Server:
         o.min_active = min_active
         o.sigma_k_target = sigma_k_target
         o.use_graph = use_graph
+        o.method = method
         self._h = C.c_void_p()
         _check(lib().nraps_server_create(C.byref(o), C.byref(self._h)))
         self._keep = {}

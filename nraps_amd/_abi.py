@@ -21,7 +21,7 @@ NRAPS_E_IO = -10
 NRAPS_E_NOT_CONVERGED = -11
 
 FP64, FP32 = 0, 1
-HISTORY_SURFACE, EVENT_WOODCOCK = 0, 1
+HISTORY_SURFACE, EVENT_WOODCOCK, EVENT_REGION = 0, 1, 2
 SOURCE_UNIFORM_FUEL, SOURCE_FISSION_BANK = 0, 1
 XS_ON_MATERIAL_CHANGE, XS_EVERY_COLLISION = 0, 1
 BANK_INIT_UNIFORM_FUEL, BANK_INIT_DIFFUSION = 0, 1
@@ -97,7 +97,7 @@ class ServerOpts(C.Structure):
         ("device", C.c_int32), ("precision", C.c_int32), ("budget_bytes", C.c_int64),
         ("max_slots", C.c_int32), ("min_active", C.c_int32),
         ("sigma_k_target", C.c_double), ("use_graph", C.c_int32),
-        ("reserved", C.c_int32 * 7),
+        ("method", C.c_int32), ("reserved", C.c_int32 * 6),
     ]
 
 

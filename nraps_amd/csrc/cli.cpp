@@ -3,7 +3,7 @@
 // dispatch and plot_solution; src/main.rs:85-143, :335-364), on the GPU
 // through the C ABI.
 //
-//   nraps_run DECK [OUTDIR] [--mc|--diffusion] [--fp32|--fp64] [--event|--plain]
+//   nraps_run DECK [OUTDIR] [--mc|--diffusion] [--fp32|--fp64] [--event|--region|--plain]
 //             [--bank] [--diffusion-start] [--histories N] [--generations N]
 //             [--skip N] [--seed S] [--every-collision]
 //
@@ -27,7 +27,7 @@ namespace {
 
 int usage() {
   std::fprintf(stderr,
-               "usage: nraps_run DECK [OUTDIR] [--mc|--diffusion] [--fp32|--fp64] [--event|--plain]\n"
+               "usage: nraps_run DECK [OUTDIR] [--mc|--diffusion] [--fp32|--fp64] [--event|--region|--plain]\n"
                "                 [--bank] [--diffusion-start] [--histories N] [--generations N]\n"
                "                 [--skip N] [--seed S] [--every-collision]\n");
   return 2;
@@ -50,7 +50,7 @@ int main(int argc, char **argv) {
   nraps_run_opts o;
   nraps_run_opts_default(&o);
   o.precision = NRAPS_FP32;
-  o.method = NRAPS_EVENT_WOODCOCK;
+  o.method = NRAPS_EVENT_REGION;
   long long hist = -1, gens = -1, skip = -1, seed = -1;
   int solver = -1;  // -1: the deck's Solution, 0: Monte Carlo, 1: diffusion
   std::vector<std::string> pos;
@@ -67,6 +67,7 @@ int main(int argc, char **argv) {
     else if (a == "--fp32") o.precision = NRAPS_FP32;
     else if (a == "--fp64") o.precision = NRAPS_FP64;
     else if (a == "--event") o.method = NRAPS_EVENT_WOODCOCK;
+    else if (a == "--region") o.method = NRAPS_EVENT_REGION;
     else if (a == "--plain") o.method = NRAPS_HISTORY_SURFACE;
     else if (a == "--bank") o.source = NRAPS_SOURCE_FISSION_BANK;
     else if (a == "--every-collision") o.xs_refresh = NRAPS_XS_EVERY_COLLISION;

@@ -84,7 +84,8 @@ int validate(const nraps_problem *pb, const nraps_run_opts *o, std::string *why)
   if (o->skip < 1 || o->skip >= o->generations)
     return bad("skip must satisfy 1 <= skip < generations");
   if (o->precision != NRAPS_FP64 && o->precision != NRAPS_FP32) return bad("unknown precision");
-  if (o->method != NRAPS_HISTORY_SURFACE && o->method != NRAPS_EVENT_WOODCOCK)
+  if (o->method != NRAPS_HISTORY_SURFACE && o->method != NRAPS_EVENT_WOODCOCK &&
+      o->method != NRAPS_EVENT_REGION)
     return bad("unknown method");
   if (o->method == NRAPS_HISTORY_SURFACE && o->precision != NRAPS_FP64)
     return bad("the plain surface-tracking port runs in FP64 only");
@@ -188,10 +189,12 @@ void Prepared::build(const nraps_problem *pb) {
   R++;
   reg_lo.assign(R, 0.0);
   reg_mat.assign(R, 0);
+  reg_cell.assign(R + 1, nc);
   std::vector<double> reg_hi(R, 0.0);
   for (int c = 0; c < nc; c++) {
     int r = region[c];
     if (c == 0 || region[c - 1] != r) {
+      reg_cell[r] = c;
       reg_lo[r] = pb->mesh_left[c];
       reg_mat[r] = mat[c];
     }
@@ -248,6 +251,8 @@ TabLayout Prepared::layout(bool fp32) const {
   L.reglo = put(8 * R);
   L.kreg = put(8 * G * (R + 1));
   L.bucket = put(2 * nb);
+  L.regedge = put(rs * (R + 1));
+  L.regcell = put(2 * (R + 1));
   L.hot = at;
   L.fuel = put(4 * nf);
   L.dx = put(rs * nc);
@@ -290,6 +295,13 @@ std::vector<uint8_t> Prepared::pack(bool fp32, TabLayout *lay) const {
   putd(L.reglo, reg_lo);
   putd(L.kreg, kreg);
   std::memcpy(&blob[L.bucket], bucket.data(), 2 * bucket.size());
+  std::vector<double> regedge(R + 1);
+  for (int r = 0; r <= R; r++) {
+    regedge[r] = edges[reg_cell[r]];
+    uint16_t c = (uint16_t)reg_cell[r];
+    std::memcpy(&blob[L.regcell + 2 * r], &c, 2);
+  }
+  putr(L.regedge, regedge);
   if (lay) *lay = L;
   return blob;
 }
@@ -535,7 +547,7 @@ extern "C" int nraps_problem_generate(int32_t config, int32_t index, uint64_t se
         o.histories = 10000000;
         o.generations = 120;
         o.skip = 20;
-        o.method = NRAPS_EVENT_WOODCOCK;
+        o.method = NRAPS_EVENT_REGION;
         o.precision = NRAPS_FP32;
       }
     } else {
@@ -575,7 +587,7 @@ extern "C" int nraps_problem_generate(int32_t config, int32_t index, uint64_t se
       o.histories = N;
       o.generations = 120;
       o.skip = 20;
-      o.method = NRAPS_EVENT_WOODCOCK;
+      o.method = NRAPS_EVENT_REGION;
       o.precision = NRAPS_FP32;
       o.source = NRAPS_SOURCE_FISSION_BANK;
       o.problem_id = config >= 3 ? (uint64_t)index : 0;

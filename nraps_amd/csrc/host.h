@@ -41,6 +41,8 @@ struct TabLayout {
   int32_t reglo;    // double[R]    region left edges
   int32_t kreg;     // double[G*(R+1)] region prefix integrals of nu*Sigma_f
   int32_t bucket;   // u16[nb]      cell lookup: largest c with edge[c] < XL + b*h
+  int32_t regedge;  // Real[R+1]    region edges, regedge[r] = edges[regcell[r]]
+  int32_t regcell;  // u16[R+1]     first cell of each region, regcell[R] = nc
   int32_t hot;      // bytes of the LDS-staged part (all of the above)
   int32_t fuel;     // i32[nf]      source cells (birth only)
   int32_t dx;       // Real[nc]     Mesh.delta_x (birth only)
@@ -55,7 +57,7 @@ struct Prepared {
   double boundl = 0, boundr = 0, XL = 0, XR = 0, maj = 0, nut_conv = 0, inv_h = 0;
   std::vector<double> edges, dx, sigw, inv_sigtr, abs_ratio, inv_sigs, nusigf, chicdf,
       cumrow, reg_lo, kreg;
-  std::vector<int32_t> mat, region, fuel, reg_mat;
+  std::vector<int32_t> mat, region, fuel, reg_mat, reg_cell;
   std::vector<uint16_t> bucket;
 
   void build(const nraps_problem *pb);

@@ -8,10 +8,12 @@
 //   k_surface          plain FP64 port: one thread per history, cell-by-cell
 //                      surface tracking, global FP64 atomics
 //                      (src/mc_code.rs:134-257)
-//   k_woodcock<Real>   event pipeline: persistent blocks stage a problem's
+//   k_event<Real>      event pipeline: persistent blocks stage a problem's
 //                      tables in LDS; each wave claims ranges of histories with
 //                      one atomic and refills idle lanes by ballot + mbcnt;
-//                      Woodcock flights, collisions and difference-array
+//                      one event per lane and step (region surface tracking:
+//                      a collision, a region edge or a face; Woodcock: a
+//                      tentative flight), difference-array
 //                      track-length tallies in LDS (flushed once per block with
 //                      FP64 atomics); fission sites appended by wave ballot
 //                      (one global atomic per wave)
@@ -132,6 +134,8 @@ struct Tabs {
   const int32_t *fuel;
   const double *nsfd, *reglo, *kreg;
   const uint16_t *bucket;
+  const Real *regedge;
+  const uint16_t *regcell;
   // The offsets are wave-uniform: with U, uni() keeps them in scalar
   // registers (the base keeps its address space, so LDS tables stay LDS
   // accesses).
@@ -153,6 +157,8 @@ struct Tabs {
     reglo = (const double *)(base + uni_if<U>(L.reglo));
     kreg = (const double *)(base + uni_if<U>(L.kreg));
     bucket = (const uint16_t *)(base + uni_if<U>(L.bucket));
+    regedge = (const Real *)(base + uni_if<U>(L.regedge));
+    regcell = (const uint16_t *)(base + uni_if<U>(L.regcell));
   }
   __device__ int mat(int c) const { return (int)mat8[c]; }
   __device__ int reg(int c) const { return (int)reg16[c]; }
@@ -278,6 +284,8 @@ struct Particle {
   // the Woodcock acceptance XS there
   int32_t xs;
   Real sw;
+  // region tracking: displacement left of the current flight
+  Real ds;
   int32_t nsites, tr;
   bool traced;
 };
@@ -735,6 +743,77 @@ __device__ inline uint32_t woodcock_step(const Ctx<Real> &C, Acc *direct, Acc *d
   return kEvStep | kEvReal;
 }
 
+// One event of surface tracking taken region by region: the reference's
+// particle_travel (src/mc_code.rs:134-213) with the cell crossings inside
+// one material region merged, since they change neither the flight nor the
+// XS.  The flight left in q.ds either ends inside the region (a collision;
+// the cell is looked up there), or reaches the region's edge in the
+// direction of travel: a face of the slab (reflection keeping the remaining
+// displacement, src/mc_code.rs:55-62, or leak) or the edge of the next
+// region (XS index refreshed to the current group, src/mc_code.rs:147, and a
+// new flight).  A new flight is also sampled after a scatter, with the XS
+// index the particle holds (stale after a group change, as in the
+// reference).  The straight path is scored where it changes direction, as
+// in Woodcock tracking.  Returns kEv* bits; face and region crossings are
+// added to `ncross`.
+template <class F, typename Real, typename Acc>
+__device__ inline uint32_t region_step(const Ctx<Real> &C, Acc *direct, Acc *diff, LaneRng &rng,
+                                       Particle<Real> &q, uint32_t &ncross, int &nsite) {
+  const Tabs<Real> &T = C.T;
+  if (++q.events > C.max_events) {
+    score_segment<Real, Acc>(C, direct, diff, q, q.x, q.cell);
+    trace_event<F>(C, q, NRAPS_EV_CAP, q.cell, q.g, 0, q.x, q.mu);
+    return kEvCap;
+  }
+  const Real x = q.x, mu = q.mu;
+  const Real end = x + q.ds;
+  const bool fwd = mu >= Real(0);
+  const int r = q.rxs;
+  const Real edge = T.regedge[fwd ? r + 1 : r];
+  uint32_t ev = kEvStep;
+  if (fwd ? end > edge : edge > end) {
+    ncross++;
+    if (fwd ? r == C.R - 1 : r == 0) {
+      const int fc = fwd ? C.nc - 1 : 0;
+      score_segment<Real, Acc>(C, direct, diff, q, edge, fc);
+      const Real b = fwd ? C.br : C.bl;
+      if (!(b > Real(0))) {
+        trace_event<F>(C, q, NRAPS_EV_LEAK, fc, q.g, 0, edge, mu);
+        return kEvStep | kEvLeak;
+      }
+      q.mu = mu * (-b);
+      q.ds = (q.ds + (x - edge)) * (-b);
+      q.x = edge;
+      q.cell = fc;
+      return kEvStep;
+    }
+    const int rn = fwd ? r + 1 : r - 1;
+    q.x = edge;
+    q.cell = fwd ? (int)T.regcell[rn] : (int)T.regcell[r] - 1;
+    q.gxs = q.g;
+    set_xs(C, q, rn, q.cell);
+  } else {
+    if (end != x) {
+      q.cell = locate(C, end, fwd);
+      q.x = end;
+    }
+    score_segment<Real, Acc>(C, direct, diff, q, q.x, q.cell);
+    const int g = q.g;
+    if (!collide<F, Real>(C, q.xs, q.xs - C.M * q.gxs, rng, q, nsite))
+      return kEvStep | kEvReal | kEvAbsorb;
+    if (q.g != g) {
+      q.ks = kint(C, q.g, q.x, q.cell);
+      if (C.xs_every) {
+        q.gxs = q.g;
+        set_xs(C, q, r, q.cell);
+      }
+    }
+    ev |= kEvReal;
+  }
+  q.ds = q.mu * -Num<Real>::nlog(Num<Real>::flight(rng.next())) * T.invtr[q.xs];
+  return ev;
+}
+
 // Per-wave counters in scalar registers: every update is the population
 // count of a ballot taken in wave-uniform control flow.
 struct WaveStats {
@@ -746,7 +825,8 @@ struct WaveStats {
     leak += __popcll(__ballot(ev & kEvLeak));
     cap += __popcll(__ballot(ev & kEvCap));
   }
-  __device__ void flush(const DevProblem &P, uint32_t ncross) {
+  // `region`: region tracking has no virtual collisions
+  __device__ void flush(const DevProblem &P, uint32_t ncross, bool region) {
     unsigned long long x = ncross;
     for (int off = 32; off >= 1; off >>= 1) x += __shfl_xor(x, off);
     if (lane_id() == 0) {
@@ -757,7 +837,7 @@ struct WaveStats {
       if (cap) atomicAdd(&P.stats[kStCapped], (unsigned long long)cap);
       if (x) atomicAdd(&P.stats[kStCrossings], x);
       // virtual collisions: flights that were neither real nor a leak
-      const uint64_t virt = steps - real - leak;
+      const uint64_t virt = region ? 0 : steps - real - leak;
       if (virt) atomicAdd(&P.stats[kStVirtual], (unsigned long long)virt);
     }
   }
@@ -772,12 +852,24 @@ struct WaveStats {
 //     descriptor in scalar registers), 0.6% faster than 5 waves; with the
 //     descriptor in vector registers it needed 128 VGPRs and spilled;
 //   1024-thread blocks run 4 waves per SIMD whatever the count; the FP64
-//     and trace variants keep the allocator's choice (at least 3 waves).
+//     and trace variants keep the allocator's choice (at least 3 waves);
+//   region tracking, FP32, 256-thread blocks: 3, 4 and 5 waves within 0.5%
+//     of each other (5 kept); 6 waves is 35% slower; with fission sites 3,
+//     4 and 5 waves within 0.5% (4 kept).
 #ifndef NRAPS_WC_WAVES_PER_EU
 #define NRAPS_WC_WAVES_PER_EU 3
 #endif
 #ifndef NRAPS_WC_HOT_UNI_TABLES
 #define NRAPS_WC_HOT_UNI_TABLES 0
+#endif
+#ifndef NRAPS_RS_HOT_VGPR
+#define NRAPS_RS_HOT_VGPR 0
+#endif
+#ifndef NRAPS_RS_FP32_WAVES
+#define NRAPS_RS_FP32_WAVES NRAPS_WC_FP32_WAVES
+#endif
+#ifndef NRAPS_RS_FP32_FISSION_WAVES
+#define NRAPS_RS_FP32_FISSION_WAVES NRAPS_WC_FP32_FISSION_WAVES
 #endif
 #ifndef NRAPS_WC_FP32_WAVES
 #define NRAPS_WC_FP32_WAVES 5
@@ -785,14 +877,16 @@ struct WaveStats {
 #ifndef NRAPS_WC_FP32_FISSION_WAVES
 #define NRAPS_WC_FP32_FISSION_WAVES 4
 #endif
-template <typename Real, bool TRACE, bool FISSION, int BS>
+template <typename Real, bool TRACE, bool FISSION, int BS, bool RS>
 constexpr int wc_waves_per_eu() {
   return (sizeof(Real) == 4 && !TRACE && BS == 256)
-             ? (FISSION ? NRAPS_WC_FP32_FISSION_WAVES : NRAPS_WC_FP32_WAVES)
+             ? (RS ? (FISSION ? NRAPS_RS_FP32_FISSION_WAVES : NRAPS_RS_FP32_WAVES)
+                   : (FISSION ? NRAPS_WC_FP32_FISSION_WAVES : NRAPS_WC_FP32_WAVES))
              : NRAPS_WC_WAVES_PER_EU;
 }
 
-// Persistent event kernel.  A block visits the problems of the batch in
+// Persistent event kernel (RS: region surface tracking, else Woodcock
+// delta tracking).  A block visits the problems of the batch in
 // cyclic order starting at its own index; on a problem with histories left
 // it stages the tables in LDS, zeroes its LDS tally, and its waves run
 // histories until the problem's queue is empty: each wave claims `claim`
@@ -804,10 +898,10 @@ constexpr int wc_waves_per_eu() {
 // size: 256 for small LDS footprints (several blocks per CU), 1024 when one
 // block per CU uses the whole LDS.
 // Dynamic LDS: [hot tables][tally direct|diff].
-template <typename Real, bool LDS_TALLY, bool TRACE, bool FISSION, int BS>
+template <typename Real, bool LDS_TALLY, bool TRACE, bool FISSION, int BS, bool RS>
 __global__ __launch_bounds__(BS)
-__attribute__((amdgpu_waves_per_eu(wc_waves_per_eu<Real, TRACE, FISSION, BS>())))
-void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim, int refill_min) {
+__attribute__((amdgpu_waves_per_eu(wc_waves_per_eu<Real, TRACE, FISSION, BS, RS>())))
+void k_event(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim, int refill_min) {
   using F = Feat<TRACE, FISSION>;
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int s_has_work;
@@ -841,10 +935,13 @@ void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim
 
     Ctx<Real> C;
     // descriptor in scalar registers for every variant but the 256-thread
-    // FP32 one without fission sites: that one measured 0.8% faster with it
-    // in vector registers (96 VGPRs, a few spills) than in scalar registers
-    // (83 VGPRs), while the fission variant gained 6% from scalar registers
-    constexpr bool kHot = sizeof(Real) == 4 && !FISSION && !TRACE && BS == 256;
+    // FP32 Woodcock one without fission sites: that one measured 0.8% faster
+    // with it in vector registers (96 VGPRs, a few spills) than in scalar
+    // registers (83 VGPRs), while the fission variant gained 6% from scalar
+    // registers and the region-tracking variant 7% (1.81 vs 1.94 ms per
+    // launch of 10^7 histories on the 9-region slab)
+    constexpr bool kHot =
+        sizeof(Real) == 4 && !FISSION && !TRACE && BS == 256 && (!RS || NRAPS_RS_HOT_VGPR);
     C.template load<!kHot, !kHot || NRAPS_WC_HOT_UNI_TABLES>(P, cs, smem);
     Acc *direct = LDS_TALLY ? (Acc *)ltally : (Acc *)P.tally_glob;
     Acc *diff = direct + C.G * C.nc;
@@ -886,6 +983,7 @@ void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim
             q.scell = q.cell;
             q.ks = kint(C, q.g, q.x, q.cell);
             trace_event<F>(C, q, NRAPS_EV_BIRTH, q.cell, q.g, 0, q.x, q.mu);
+            if (RS) q.ds = q.mu * -Num<Real>::nlog(Num<Real>::flight(rng.next())) * C.T.invtr[q.xs];
             active = true;
           }
           r_cur += take;
@@ -898,7 +996,12 @@ void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim
       }
       uint32_t ev = 0;
       int nsite = 0;
-      if (active) ev = woodcock_step<F, Real, Acc>(C, direct, diff, rng, q, ncross, nsite);
+      if (active) {
+        if constexpr (RS)
+          ev = region_step<F, Real, Acc>(C, direct, diff, rng, q, ncross, nsite);
+        else
+          ev = woodcock_step<F, Real, Acc>(C, direct, diff, rng, q, ncross, nsite);
+      }
       if (F::FISSION && C.make_sites) sites.emit(C, nsite, q);  // wave-uniform
       if (active && (ev & kEvDead)) {
         finish<F>(C, q, rng);
@@ -907,7 +1010,7 @@ void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim
       ws.add(ev);
     }
     if (F::FISSION && C.make_sites) sites.close(C);
-    ws.flush(P, ncross);
+    ws.flush(P, ncross, RS);
     __syncthreads();
     if (LDS_TALLY) {
       for (int i = threadIdx.x; i < TB; i += BS) {
@@ -919,12 +1022,14 @@ void k_woodcock(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim
   }
 }
 
-#define NRAPS_WC_INST(R, L, BS)                                                                \
-  template __global__ void k_woodcock<R, L, false, false, BS>(const DevProblem *, int, int, int, \
+#define NRAPS_WC_INST1(R, L, T, Fi, BS)                                                     \
+  template __global__ void k_event<R, L, T, Fi, BS, false>(const DevProblem *, int, int, int, \
                                                               int);                             \
-  template __global__ void k_woodcock<R, L, false, true, BS>(const DevProblem *, int, int, int,  \
-                                                             int);                              \
-  template __global__ void k_woodcock<R, L, true, true, BS>(const DevProblem *, int, int, int, int);
+  template __global__ void k_event<R, L, T, Fi, BS, true>(const DevProblem *, int, int, int, int);
+#define NRAPS_WC_INST(R, L, BS)            \
+  NRAPS_WC_INST1(R, L, false, false, BS) \
+  NRAPS_WC_INST1(R, L, false, true, BS)  \
+  NRAPS_WC_INST1(R, L, true, true, BS)
 NRAPS_WC_INST(float, true, 256)
 NRAPS_WC_INST(float, false, 256)
 NRAPS_WC_INST(float, true, 1024)
@@ -932,6 +1037,7 @@ NRAPS_WC_INST(double, true, 256)
 NRAPS_WC_INST(double, false, 256)
 NRAPS_WC_INST(double, true, 1024)
 #undef NRAPS_WC_INST
+#undef NRAPS_WC_INST1
 
 
 // ---------------------------------------------------------------------------
@@ -1245,19 +1351,20 @@ void launch_surface(const DevProblem *d, int nprob, int64_t max_n, hipStream_t s
 // Variant of the event kernel: precision, tally placement, and the
 // compile-time features (traces imply the fission code path).
 template <typename Real, bool L, bool T, bool Fi, int BS>
-static const void *wc_fn() {
-  return reinterpret_cast<const void *>(&k_woodcock<Real, L, T, Fi, BS>);
+static const void *wc_fn(bool region) {
+  return region ? reinterpret_cast<const void *>(&k_event<Real, L, T, Fi, BS, true>)
+                : reinterpret_cast<const void *>(&k_event<Real, L, T, Fi, BS, false>);
 }
 template <typename Real, bool L, int BS>
-static const void *wc_pick(bool trace, bool fission) {
-  if (trace) return wc_fn<Real, L, true, true, BS>();
-  return fission ? wc_fn<Real, L, false, true, BS>() : wc_fn<Real, L, false, false, BS>();
+static const void *wc_pick(bool trace, bool fission, bool region) {
+  if (trace) return wc_fn<Real, L, true, true, BS>(region);
+  return fission ? wc_fn<Real, L, false, true, BS>(region) : wc_fn<Real, L, false, false, BS>(region);
 }
 template <typename Real>
 static const void *wc_pick_mode(const WcVariant &v) {
-  if (v.big_block) return wc_pick<Real, true, 1024>(v.trace, v.fission);
-  return v.lds_tally ? wc_pick<Real, true, 256>(v.trace, v.fission)
-                     : wc_pick<Real, false, 256>(v.trace, v.fission);
+  if (v.big_block) return wc_pick<Real, true, 1024>(v.trace, v.fission, v.region);
+  return v.lds_tally ? wc_pick<Real, true, 256>(v.trace, v.fission, v.region)
+                     : wc_pick<Real, false, 256>(v.trace, v.fission, v.region);
 }
 static const void *wc_select(const WcVariant &v) {
   return v.fp32 ? wc_pick_mode<float>(v) : wc_pick_mode<double>(v);

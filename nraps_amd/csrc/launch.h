@@ -16,6 +16,7 @@ struct WcVariant {
   bool big_block = false; // one 1024-thread block per CU owning the whole LDS
   bool trace = false;     // per-history parity traces compiled in
   bool fission = false;   // fission-site banking compiled in
+  bool region = false;    // region-by-region surface tracking (else Woodcock)
   int block_size() const { return big_block ? 1024 : 256; }
 };
 void launch_woodcock(const WcVariant &v, const DevProblem *d, int nprob, int grid, size_t lds_bytes,

@@ -502,6 +502,7 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     variant_.lds_tally = lds_tally_;
     variant_.fission = fission_any_;
     variant_.trace = trace_any;
+    variant_.region = ps[0]->o.method == NRAPS_EVENT_REGION;  // one method per batch
     (void)woodcock_set_lds_limit(variant_, lds_bytes_);
     blocks_per_cu_ = woodcock_blocks_per_cu(variant_, lds_bytes_);
     grid_ = n_cu_ * blocks_per_cu_;
@@ -647,7 +648,7 @@ int solve_batch(int n, const nraps_problem *const *pbs, const nraps_run_opts *op
     }
     HIP_TRY(hipSetDevice(device));
     const bool fp32 = opts[0].precision == NRAPS_FP32;
-    const bool woodcock = opts[0].method == NRAPS_EVENT_WOODCOCK;
+    const bool woodcock = opts[0].method != NRAPS_HISTORY_SURFACE;  // event pipeline
     const bool use_graph = opts[0].use_graph != 0;
     // the stream outlives the problems: their memory is freed on it
     StreamGuard sg;

@@ -336,8 +336,9 @@ extern "C" int nraps_server_create(const nraps_server_opts *o, nraps_server **ou
   }
   *out = nullptr;
   if (o->max_slots < 1 || o->budget_bytes <= 0 ||
-      (o->precision != NRAPS_FP32 && o->precision != NRAPS_FP64)) {
-    set_error("server needs max_slots >= 1, budget_bytes > 0 and a valid precision");
+      (o->precision != NRAPS_FP32 && o->precision != NRAPS_FP64) ||
+      (o->method != 0 && o->method != NRAPS_EVENT_REGION && o->method != NRAPS_EVENT_WOODCOCK)) {
+    set_error("server needs max_slots >= 1, budget_bytes > 0, a valid precision and an event method");
     return NRAPS_E_INVALID;
   }
   int ndev = 0;
@@ -370,7 +371,7 @@ extern "C" int nraps_server_submit(nraps_server *srv, const nraps_problem *pb, c
     return NRAPS_E_INVALID;
   }
   nraps_run_opts oo = *o;
-  oo.method = NRAPS_EVENT_WOODCOCK;
+  oo.method = srv->opts.method == NRAPS_EVENT_WOODCOCK ? NRAPS_EVENT_WOODCOCK : NRAPS_EVENT_REGION;
   oo.precision = srv->opts.precision;
   oo.trace_histories = 0;
   std::string why;

@@ -41,7 +41,9 @@ def main():
     ap.add_argument("--generations", type=int, default=4)
     ap.add_argument("--histories", type=int, default=0)
     ap.add_argument("--precision", choices=["fp32", "fp64"], default=None)
-    ap.add_argument("--method", choices=["event", "plain"], default=None)
+    ap.add_argument("--method", choices=["event", "region", "plain"], default=None,
+                    help="event: Woodcock event kernel; region: region surface-tracking "
+                         "event kernel; plain: FP64 one-thread-per-history port")
     ap.add_argument("--copy-mib", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0, help="histories per wave claim (0: automatic)")
     ap.add_argument("--bank", action="store_true", help="fission-bank source (skip >= 2)")
@@ -73,7 +75,8 @@ def main():
     if a.precision:
         o.precision = _abi.FP32 if a.precision == "fp32" else _abi.FP64
     if a.method:
-        o.method = _abi.EVENT_WOODCOCK if a.method == "event" else _abi.HISTORY_SURFACE
+        o.method = {"event": _abi.EVENT_WOODCOCK, "region": _abi.EVENT_REGION,
+                    "plain": _abi.HISTORY_SURFACE}[a.method]
         if o.method == _abi.HISTORY_SURFACE:
             o.precision = _abi.FP64
     r = nr.solve(p, o)

@@ -74,13 +74,13 @@ def percentile(xs, q):
 
 def run_serving(rate, seconds, *, first_index=0, sigma_pcm=50.0, min_active=10,
                 budget_bytes=200 * 10**9, max_slots=1024, drain_s=600.0, precision=_abi.FP32,
-                use_graph=1, log=None):
+                use_graph=1, log=None, method=_abi.EVENT_REGION):
     times = arrival_times(rate, seconds)
     reqs = [nr.generate(4, first_index + i) for i in range(len(times))]
     lat, qlat, hist, cycles = [], [], 0, []
     with nr.Server(precision=precision, budget_bytes=budget_bytes, max_slots=max_slots,
                    min_active=min_active, sigma_k_target=sigma_pcm * 1e-5,
-                   use_graph=use_graph) as srv:
+                   use_graph=use_graph, method=method) as srv:
         tickets = []
         t0 = time.perf_counter()
         for (p, o), ta in zip(reqs, times):
@@ -123,12 +123,13 @@ def run_serving(rate, seconds, *, first_index=0, sigma_pcm=50.0, min_active=10,
     }
 
 
-def run_batch(n, *, use_graph=1, precision=_abi.FP32, generations=None):
+def run_batch(n, *, use_graph=1, precision=_abi.FP32, generations=None, method=_abi.EVENT_REGION):
     probs, opts = [], []
     for i in range(n):
         p, o = nr.generate(3, i)
         o.use_graph = use_graph
         o.precision = precision
+        o.method = method
         if generations:
             o.generations = generations
         probs.append(p)
@@ -161,19 +162,22 @@ def main():
     ap.add_argument("--no-graph", action="store_true",
                     help="serving: launch each cycle directly instead of replaying a hipGraph "
                          "captured while membership is unchanged")
+    ap.add_argument("--method", choices=["region", "woodcock"], default="region",
+                    help="tracking of the event kernel")
     a = ap.parse_args()
+    method = _abi.EVENT_REGION if a.method == "region" else _abi.EVENT_WOODCOCK
 
     def log(m):
         print(m, file=sys.stderr, flush=True)
 
     if a.batch:
-        out = run_batch(a.batch, generations=a.generations or None)
+        out = run_batch(a.batch, generations=a.generations or None, method=method)
         print(json.dumps({"form": "batched", **out}), flush=True)
         return
     idx = 0
     for rate in [float(x) for x in a.rates.split(",") if x]:
         out = run_serving(rate, a.seconds, first_index=idx, drain_s=a.drain,
-                          use_graph=0 if a.no_graph else 1, log=log)
+                          use_graph=0 if a.no_graph else 1, log=log, method=method)
         idx += out["submitted"]
         print(json.dumps({"form": "serving", **out}), flush=True)
 

@@ -163,7 +163,7 @@ typedef struct {
   const double *scat;
   double maj;
   int R;
-  int32_t *reg_of, *reg_mat;
+  int32_t *reg_of, *reg_mat, *reg_first; /* reg_first[r]: first cell, [R] = nc */
   double *reg_lo, *reg_hi, *kreg; /* kreg[g*(R+1) + r] */
 } oprob;
 
@@ -179,6 +179,7 @@ static void oprob_free(oprob *p) {
   free(p->reg_mat);
   free(p->reg_lo);
   free(p->reg_hi);
+  free(p->reg_first);
   free(p->kreg);
 }
 
@@ -255,11 +256,14 @@ static int oprob_init(oprob *p, const nraps_problem *pb) {
   p->reg_mat = malloc(sizeof(int32_t) * R);
   p->reg_lo = malloc(sizeof(double) * R);
   p->reg_hi = malloc(sizeof(double) * R);
+  p->reg_first = malloc(sizeof(int32_t) * (R + 1));
+  p->reg_first[R] = nc;
   for (int c = 0; c < nc; c++) {
     int r = p->reg_of[c];
     if (c == 0 || p->reg_of[c - 1] != r) {
       p->reg_lo[r] = pb->mesh_left[c];
       p->reg_mat[r] = pb->matid[c];
+      p->reg_first[r] = c;
     }
     p->reg_hi[r] = pb->mesh_right[c];
   }
@@ -613,6 +617,81 @@ static double ohistory_woodcock(const ocycle *cy, owork *w, int64_t pidx, orng *
   }
 }
 
+/* One history with the reference's surface tracking taken region by region
+   (particle_travel, src/mc_code.rs:134-213): the cell crossings inside a
+   material region are merged, since they change neither the flight nor the
+   XS index.  A flight is sampled after the birth, after a scatter (with
+   the XS index the particle holds: stale after a group change) and on entry
+   to another region (XS index refreshed to the current group,
+   src/mc_code.rs:147).  Faces reflect with the remaining displacement
+   (src/mc_code.rs:55-62) or leak.  The draws are those of the cell-by-cell
+   tracker, in the same order; only the rounding of positions differs.
+   Straight paths are scored where they end, as in ohistory_woodcock. */
+static double ohistory_region(const ocycle *cy, owork *w, int64_t pidx, orng *r) {
+  const oprob *p = cy->p;
+  const int M = p->M, nc = p->nc, R = p->R;
+  int xs_every = cy->o->xs_refresh == NRAPS_XS_EVERY_COLLISION;
+  int64_t cap = cy->o->max_events > 0 ? cy->o->max_events : 10000000;
+  double x, mu;
+  int32_t cell, g;
+  double kacc = 0.0;
+  obirth(cy, pidx, r, &x, &mu, &cell, &g);
+  otrace(w, NRAPS_EV_BIRTH, cell, g, 0, x, mu);
+  int32_t rr = p->reg_of[cell];
+  int xs = p->mat[cell] + M * g;
+  oseg seg = {x, cell};
+  double ds = mu * -nraps_log(orng_flight(r)) * p->inv_sigtr[xs];
+  int64_t events = 0;
+  for (;;) {
+    if (++events > cap) {
+      oscore(p, w, &seg, g, mu, x, cell, &kacc);
+      w->st.capped++;
+      otrace(w, NRAPS_EV_CAP, cell, g, 0, x, mu);
+      return kacc;
+    }
+    w->st.steps++;
+    double end = x + ds;
+    int fwd = mu >= 0.0;
+    /* region edges: left edge of the region's first cell; the right face */
+    int er = fwd ? rr + 1 : rr;
+    double edge = er < R ? p->left[p->reg_first[er]] : p->XR;
+    if (fwd ? end > edge : edge > end) {
+      w->st.crossings++;
+      if (fwd ? rr == R - 1 : rr == 0) {
+        int32_t fc = fwd ? nc - 1 : 0;
+        oscore(p, w, &seg, g, mu, edge, fc, &kacc);
+        double bound = fwd ? p->boundr : p->boundl;
+        if (!(bound > 0.0)) {
+          w->st.leaks++;
+          otrace(w, NRAPS_EV_LEAK, fc, g, 0, edge, mu);
+          return kacc;
+        }
+        double o3[3];
+        oracle_hit_boundary(mu, x, ds, bound, edge, o3);
+        mu = o3[0];
+        ds = o3[1];
+        x = o3[2];
+        cell = fc;
+        continue;
+      }
+      int32_t rn = fwd ? rr + 1 : rr - 1;
+      x = edge;
+      cell = fwd ? p->reg_first[rn] : p->reg_first[rr] - 1;
+      rr = rn;
+      xs = p->mat[cell] + M * g;
+    } else {
+      if (end != x) {
+        cell = olocate(p, end, mu);
+        x = end;
+      }
+      oscore(p, w, &seg, g, mu, x, cell, &kacc);
+      if (!ocollide(cy, w, r, pidx, xs, cell, x, &g, &mu)) return kacc;
+      if (xs_every) xs = p->mat[cell] + M * g;
+    }
+    ds = mu * -nraps_log(orng_flight(r)) * p->inv_sigtr[xs];
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Canonical k sum                                                           */
 /* ------------------------------------------------------------------------ */
@@ -666,7 +745,7 @@ static void orun_block(ojob *J, int64_t b) {
   w.tally = &J->block_tally[(size_t)b * p->G * p->nc];
   memset(w.tally, 0, sizeof(double) * p->G * p->nc);
   int64_t lo = b * J->block, hi = lo + J->block < J->N ? lo + J->block : J->N;
-  int woodcock = cy->o->method == NRAPS_EVENT_WOODCOCK;
+  int method = cy->o->method;
   nraps_result *res = J->res;
   for (int64_t i = lo; i < hi; i++) {
     orng r;
@@ -675,7 +754,9 @@ static void orun_block(ojob *J, int64_t b) {
     w.tr = traced && res->trace ? &res->trace[i * cy->o->trace_events] : NULL;
     w.tr_cap = traced ? cy->o->trace_events : 0;
     w.tr_len = 0;
-    double k = woodcock ? ohistory_woodcock(cy, &w, i, &r) : ohistory_surface(cy, &w, i, &r);
+    double k = method == NRAPS_EVENT_WOODCOCK ? ohistory_woodcock(cy, &w, i, &r)
+               : method == NRAPS_EVENT_REGION ? ohistory_region(cy, &w, i, &r)
+                                              : ohistory_surface(cy, &w, i, &r);
     J->kacc[i] = k;
     if (traced) {
       if (res->trace_len) res->trace_len[i] = w.tr_len;

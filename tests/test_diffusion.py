@@ -317,7 +317,8 @@ def test_diffusion_rejects_bad_input():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK,
+                                    _abi.EVENT_REGION])
 def test_diffusion_started_bank_bit_exact(gpu, oracle, method):
     """FP64 Monte Carlo from a diffusion-started bank, against the oracle run
     from the same diffusion source: every generation's k (a canonical-order

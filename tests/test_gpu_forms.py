@@ -119,7 +119,8 @@ def test_server_cancel(gpu):
 
 
 @pytest.mark.parametrize("n", [1, 63, 64, 65, 257])
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK,
+                                    _abi.EVENT_REGION])
 def test_ragged_history_counts(gpu, oracle, n, method):
     p, _ = nr.generate(0)
     o = nr.run_opts(histories=n, generations=3, skip=1, method=method, precision=_abi.FP64,
@@ -134,7 +135,8 @@ def test_ragged_history_counts(gpu, oracle, n, method):
     assert g.trace[: n * 64].tobytes() == c.trace[: n * 64].tobytes()
 
 
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK,
+                                    _abi.EVENT_REGION])
 def test_event_cap_matches_oracle(gpu, oracle, method):
     p, _ = nr.generate(0)
     o = nr.run_opts(histories=3000, generations=2, skip=1, method=method, max_events=3)
@@ -147,7 +149,8 @@ def test_event_cap_matches_oracle(gpu, oracle, method):
 
 
 @pytest.mark.parametrize("deck", ["TestCaseA", "TestCaseB", "TestCaseC"])
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK,
+                                    _abi.EVENT_REGION])
 def test_reference_decks_fp64_match_oracle(gpu, oracle, deck, method):
     """The shipped decks (2- and 4-group, reflective and vacuum faces,
     expanded by mesh_gen) through both FP64 GPU paths: k bit-identical to
@@ -189,6 +192,7 @@ def test_statistical_agreement_9region_slab(gpu, oracle):
         p, nr.run_opts(histories=100000, generations=12, skip=2, seed=s), threads=7), seeds)
     modes = {"fp32 event": dict(method=_abi.EVENT_WOODCOCK, precision=_abi.FP32),
              "fp64 event": dict(method=_abi.EVENT_WOODCOCK, precision=_abi.FP64),
+             "fp32 region": dict(method=_abi.EVENT_REGION, precision=_abi.FP32),
              "fp64 plain": dict(method=_abi.HISTORY_SURFACE, precision=_abi.FP64)}
     got = {}
     for name, kw in modes.items():
@@ -279,7 +283,7 @@ def test_command_line_driver_runs_a_reference_deck(gpu, tmp_path):
         assert (tmp_path / f).exists()
     p, _ = nr.load_deck(deck)
     rb = nr.solve(p, nr.run_opts(histories=20000, generations=8, skip=2, precision=_abi.FP32,
-                                 method=_abi.EVENT_WOODCOCK))
+                                 method=_abi.EVENT_REGION))
     k_row = (tmp_path / "k_eff.csv").read_text().splitlines()[0].split(",")
     assert len(k_row) == 8
     assert abs(float(k_row[-1]) - rb.k[-1]) <= 1e-12 * abs(rb.k[-1])

@@ -60,20 +60,25 @@ def test_plain_fp64_port_bit_exact(gpu, oracle, refresh):
         assert getattr(g, f) == getattr(c, f), f
 
 
+EVENT_METHODS = [_abi.EVENT_WOODCOCK, _abi.EVENT_REGION]
+ALL_METHODS = [_abi.HISTORY_SURFACE] + EVENT_METHODS
+
+
+@pytest.mark.parametrize("method", EVENT_METHODS)
 @pytest.mark.parametrize("refresh", [_abi.XS_ON_MATERIAL_CHANGE, _abi.XS_EVERY_COLLISION])
-def test_woodcock_fp64_bit_exact(gpu, oracle, refresh):
-    p, o = small_c0(method=_abi.EVENT_WOODCOCK, precision=_abi.FP64, xs_refresh=refresh)
+def test_event_fp64_bit_exact(gpu, oracle, refresh, method):
+    p, o = small_c0(method=method, precision=_abi.FP64, xs_refresh=refresh)
     g = nr.solve(p, o, trace=True)
     c = oracle.solve(p, o, threads=4, trace=True)
     assert_traces_equal(g, c, o.histories)
     assert np.array_equal(g.k.view(np.uint64), c.k.view(np.uint64))
     # difference-array tally on the GPU vs cell-by-cell overlaps in the oracle
     np.testing.assert_allclose(g.tally_last, c.tally_last, rtol=1e-10, atol=1e-10)
-    for f in ("collisions", "virtual_collisions", "absorptions", "leaks", "capped"):
+    for f in ("collisions", "virtual_collisions", "crossings", "absorptions", "leaks", "capped"):
         assert getattr(g, f) == getattr(c, f), f
 
 
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", ALL_METHODS)
 def test_fission_bank_fp64_bit_exact(gpu, oracle, method):
     p, _ = nr.generate(2)
     o = nr.run_opts(histories=4000, generations=6, skip=2, method=method,
@@ -87,12 +92,13 @@ def test_fission_bank_fp64_bit_exact(gpu, oracle, method):
     assert_traces_equal(g, c, o.histories)
 
 
-def test_fp32_single_histories_follow_fp64(gpu):
-    """FP32 Woodcock histories replayed in FP64 (same streams): the event
+@pytest.mark.parametrize("method", EVENT_METHODS)
+def test_fp32_single_histories_follow_fp64(gpu, method):
+    """FP32 event-kernel histories replayed in FP64 (same streams): the event
     sequences agree for >= 99% of histories and the positions of agreeing
     events to 2e-5 cm (slab of 7 cm, FP32 ulp ~5e-7)."""
-    p, o64 = small_c0(histories=20000, method=_abi.EVENT_WOODCOCK, precision=_abi.FP64)
-    _, o32 = small_c0(histories=20000, method=_abi.EVENT_WOODCOCK, precision=_abi.FP32)
+    p, o64 = small_c0(histories=20000, method=method, precision=_abi.FP64)
+    _, o32 = small_c0(histories=20000, method=method, precision=_abi.FP32)
     a = nr.solve(p, o64, trace=True)
     b = nr.solve(p, o32, trace=True)
     same, worst = 0, 0.0
@@ -107,7 +113,7 @@ def test_fp32_single_histories_follow_fp64(gpu):
     assert worst <= 2e-5
 
 
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", ALL_METHODS)
 def test_partial_albedo_faces_bit_exact(gpu, oracle, method):
     """Albedo faces between vacuum and mirror (src/mc_code.rs:55-62: mu -> -b*mu
     and the remaining displacement scaled by -b), b = 0.5 on the left and

@@ -87,7 +87,8 @@ def run(oracle, p, **kw):
     return r, o
 
 
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK,
+                                    _abi.EVENT_REGION])
 @pytest.mark.parametrize("upscatter", [False, True])
 def test_infinite_medium_k(oracle, method, upscatter):
     p, x = homogeneous(upscatter=upscatter)
@@ -95,7 +96,8 @@ def test_infinite_medium_k(oracle, method, upscatter):
     assert abs(r.k_mean - k_infinite(x)) < 4 * r.k_sigma
 
 
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK,
+                                    _abi.EVENT_REGION])
 def test_stale_xs_markov_chain(oracle, method):
     p, x = homogeneous()
     kst, kinf = k_stale_chain(x), k_infinite(x)
@@ -104,7 +106,8 @@ def test_stale_xs_markov_chain(oracle, method):
     assert abs(r.k_mean - kst) < 4 * r.k_sigma
 
 
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK,
+                                    _abi.EVENT_REGION])
 def test_fission_bank_fundamental_mode_is_k_infinite(oracle, method):
     p, x = homogeneous(upscatter=True)
     r, _ = run(oracle, p, method=method, generations=12, skip=3, histories=30000,
@@ -126,7 +129,8 @@ def test_escape_probability_formula():
         assert abs(val - escape_probability(tau)) < 1e-7
 
 
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK,
+                                    _abi.EVENT_REGION])
 @pytest.mark.parametrize("tau", [0.5, 2.0])
 def test_pure_absorber_slab_leakage(oracle, method, tau):
     sig, L = 0.8, tau / 0.8
@@ -149,7 +153,8 @@ def test_results_do_not_depend_on_thread_count(oracle):
         assert np.array_equal(getattr(a, f), getattr(b, f)), f
 
 
-@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("method", [_abi.HISTORY_SURFACE, _abi.EVENT_WOODCOCK,
+                                    _abi.EVENT_REGION])
 def test_conservation_and_k_from_tally(oracle, method):
     p, _ = nr.generate(0)
     r, o = run(oracle, p, method=method, histories=20000, generations=3)
