@@ -248,10 +248,9 @@ TabLayout Prepared::layout(bool fp32) const {
   L.chicdf = put(rs * GM);
   L.cumrow = put(rs * M * G * G);
   L.nsfd = put(8 * GM);
-  L.reglo = put(8 * R);
   L.kreg = put(8 * G * (R + 1));
   L.bucket = put(2 * nb);
-  L.regedge = put(rs * (R + 1));
+  L.regedge = put(8 * (R + 1));
   L.regcell = put(2 * (R + 1));
   L.hot = at;
   L.fuel = put(4 * nf);
@@ -292,7 +291,6 @@ std::vector<uint8_t> Prepared::pack(bool fp32, TabLayout *lay) const {
   putr(L.chicdf, chicdf);
   putr(L.cumrow, cumrow);
   putd(L.nsfd, nusigf);
-  putd(L.reglo, reg_lo);
   putd(L.kreg, kreg);
   std::memcpy(&blob[L.bucket], bucket.data(), 2 * bucket.size());
   std::vector<double> regedge(R + 1);
@@ -301,7 +299,7 @@ std::vector<uint8_t> Prepared::pack(bool fp32, TabLayout *lay) const {
     uint16_t c = (uint16_t)reg_cell[r];
     std::memcpy(&blob[L.regcell + 2 * r], &c, 2);
   }
-  putr(L.regedge, regedge);
+  putd(L.regedge, regedge);
   if (lay) *lay = L;
   return blob;
 }

@@ -38,10 +38,9 @@ struct TabLayout {
   int32_t chicdf;   // Real[M*G]    cumulative chi per material
   int32_t cumrow;   // Real[M*G*G]  cumulative scatter rows
   int32_t nsfd;     // double[G*M]  nut*sigf (k accumulation)
-  int32_t reglo;    // double[R]    region left edges
   int32_t kreg;     // double[G*(R+1)] region prefix integrals of nu*Sigma_f
   int32_t bucket;   // u16[nb]      cell lookup: largest c with edge[c] < XL + b*h
-  int32_t regedge;  // Real[R+1]    region edges, regedge[r] = edges[regcell[r]]
+  int32_t regedge;  // double[R+1]  region edges, regedge[r] = edges[regcell[r]]
   int32_t regcell;  // u16[R+1]     first cell of each region, regcell[R] = nc
   int32_t hot;      // bytes of the LDS-staged part (all of the above)
   int32_t fuel;     // i32[nf]      source cells (birth only)

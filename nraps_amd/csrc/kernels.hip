@@ -132,9 +132,9 @@ struct Tabs {
   const uint8_t *mat8;
   const uint16_t *reg16;
   const int32_t *fuel;
-  const double *nsfd, *reglo, *kreg;
+  const double *nsfd, *kreg;
   const uint16_t *bucket;
-  const Real *regedge;
+  const double *regedge;  // region edges in FP64: the k integral's origins too
   const uint16_t *regcell;
   // The offsets are wave-uniform: with U, uni() keeps them in scalar
   // registers (the base keeps its address space, so LDS tables stay LDS
@@ -154,10 +154,9 @@ struct Tabs {
     chicdf = (const Real *)(base + uni_if<U>(L.chicdf));
     cumrow = (const Real *)(base + uni_if<U>(L.cumrow));
     nsfd = (const double *)(base + uni_if<U>(L.nsfd));
-    reglo = (const double *)(base + uni_if<U>(L.reglo));
     kreg = (const double *)(base + uni_if<U>(L.kreg));
     bucket = (const uint16_t *)(base + uni_if<U>(L.bucket));
-    regedge = (const Real *)(base + uni_if<U>(L.regedge));
+    regedge = (const double *)(base + uni_if<U>(L.regedge));
     regcell = (const uint16_t *)(base + uni_if<U>(L.regcell));
   }
   __device__ int mat(int c) const { return (int)mat8[c]; }
@@ -633,7 +632,7 @@ template <typename Real>
 __device__ inline double kint(const Ctx<Real> &C, int g, Real x, int c) {
   const Tabs<Real> &T = C.T;
   int r = T.reg(c);
-  return T.kreg[g * (C.R + 1) + r] + T.nsfd[T.mat(c) + C.M * g] * ((double)x - T.reglo[r]);
+  return T.kreg[g * (C.R + 1) + r] + T.nsfd[T.mat(c) + C.M * g] * ((double)x - T.regedge[r]);
 }
 
 // XS refresh: the particle's XS group gxs is taken in region r (material of
@@ -769,7 +768,7 @@ __device__ inline uint32_t region_step(const Ctx<Real> &C, Acc *direct, Acc *dif
   const Real end = x + q.ds;
   const bool fwd = mu >= Real(0);
   const int r = q.rxs;
-  const Real edge = T.regedge[fwd ? r + 1 : r];
+  const Real edge = (Real)T.regedge[fwd ? r + 1 : r];
   uint32_t ev = kEvStep;
   if (fwd ? end > edge : edge > end) {
     ncross++;
@@ -897,16 +896,15 @@ constexpr int wc_waves_per_eu() {
 // LDS are scored straight into the global FP64 tally.  BS is the block
 // size: 256 for small LDS footprints (several blocks per CU), 1024 when one
 // block per CU uses the whole LDS.
-// Dynamic LDS: [hot tables][tally direct|diff].
+// Dynamic LDS: [hot tables of the problem][its tally direct|diff].
 template <typename Real, bool LDS_TALLY, bool TRACE, bool FISSION, int BS, bool RS>
 __global__ __launch_bounds__(BS)
 __attribute__((amdgpu_waves_per_eu(wc_waves_per_eu<Real, TRACE, FISSION, BS, RS>())))
-void k_event(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim, int refill_min) {
+void k_event(const DevProblem *probs, int nprob, int claim, int refill_min) {
   using F = Feat<TRACE, FISSION>;
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int s_has_work;
   typedef typename std::conditional<LDS_TALLY, Real, double>::type Acc;
-  Real *ltally = (Real *)(smem + tab_lds_bytes);
   int loaded = -1;
   for (int k = 0; k < nprob; k++) {
     const int pi = (int)((blockIdx.x + (unsigned)k) % (unsigned)nprob);
@@ -929,6 +927,9 @@ void k_event(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim, i
       loaded = pi;
     }
     const int TB = P.tally_bins;
+    // the tally follows this problem's tables (the dynamic LDS holds the
+    // largest tables + tally of any one problem of the batch)
+    Real *ltally = (Real *)(smem + uni(P.lay.hot));
     if (LDS_TALLY)
       for (int i = threadIdx.x; i < TB; i += BS) ltally[i] = Real(0);
     __syncthreads();
@@ -1023,9 +1024,8 @@ void k_event(const DevProblem *probs, int nprob, int tab_lds_bytes, int claim, i
 }
 
 #define NRAPS_WC_INST1(R, L, T, Fi, BS)                                                     \
-  template __global__ void k_event<R, L, T, Fi, BS, false>(const DevProblem *, int, int, int, \
-                                                              int);                             \
-  template __global__ void k_event<R, L, T, Fi, BS, true>(const DevProblem *, int, int, int, int);
+  template __global__ void k_event<R, L, T, Fi, BS, false>(const DevProblem *, int, int, int); \
+  template __global__ void k_event<R, L, T, Fi, BS, true>(const DevProblem *, int, int, int);
 #define NRAPS_WC_INST(R, L, BS)            \
   NRAPS_WC_INST1(R, L, false, false, BS) \
   NRAPS_WC_INST1(R, L, false, true, BS)  \
@@ -1371,8 +1371,8 @@ static const void *wc_select(const WcVariant &v) {
 }
 
 void launch_woodcock(const WcVariant &v, const DevProblem *d, int nprob, int grid, size_t lds_bytes,
-                     int tab_lds_bytes, int claim, int refill_min, hipStream_t s) {
-  void *args[] = {(void *)&d, (void *)&nprob, (void *)&tab_lds_bytes, (void *)&claim,
+                     int claim, int refill_min, hipStream_t s) {
+  void *args[] = {(void *)&d, (void *)&nprob, (void *)&claim,
                   (void *)&refill_min};
   hipError_t e = hipLaunchKernel(wc_select(v), dim3((unsigned)grid), dim3(v.block_size()), args,
                                  lds_bytes, s);

@@ -131,7 +131,7 @@ class Launcher {
   void drop_graph();
   int device_;
   int n_ = 0, n_cu_ = 0, lds_limit_ = 0, blocks_per_cu_ = 1, grid_ = 1, claim_ = 256, refill_min_ = 1;
-  int tab_lds_ = 0, max_bins_ = 0, levels_ = 1, max_groups_ = 1;
+  int max_bins_ = 0, levels_ = 1, max_groups_ = 1;
   int64_t max_n_ = 0, max_tiles_ = 0;
   size_t lds_bytes_ = 0;
   bool fp32_ = false, woodcock_ = false, fission_any_ = false, lds_tally_ = false;

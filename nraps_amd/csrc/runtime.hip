@@ -453,7 +453,7 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
   max_groups_ = 1;
   levels_ = 1;
   max_tiles_ = 0;
-  int tab_max = 0, tally_max = 0;
+  int tab_max = 0, need_max = 0;  // largest tables; largest tables + tally of one problem
   bool trace_any = false;
   int64_t tiles_total = 0, total = 0;
   std::vector<int64_t> tile_base(std::max(n_, 1));
@@ -470,14 +470,16 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     tiles_total += t;
     max_tiles_ = std::max(max_tiles_, t);
     tab_max = std::max(tab_max, p.lay.hot);
-    tally_max = std::max(tally_max, p.dev.tally_bins * (fp32 ? 4 : 8));
+    need_max = std::max(need_max, p.lay.hot + p.dev.tally_bins * (fp32 ? 4 : 8));
     total += p.o.histories;
   }
   if (woodcock_ && n_ > 0) {
-    // LDS plan.  The hot tables always go to LDS.  The tally goes there too:
-    // with 256-thread blocks when tables + tally leave room for four blocks
-    // per CU, else with one 1024-thread block per CU owning the whole LDS;
-    // only when neither fits is it scored into the global FP64 tally.
+    // LDS plan.  The hot tables always go to LDS.  The tally goes there too,
+    // right after the tables of the problem a block works on: with
+    // 256-thread blocks when the largest tables + tally of any one problem
+    // leave room for four blocks per CU, else with one 1024-thread block per
+    // CU owning the whole LDS; only when neither fits is it scored into the
+    // global FP64 tally.
     if (tab_max > lds_limit_)
       throw std::invalid_argument("problem tables (" + std::to_string(tab_max) +
                                   " B) exceed the LDS of one block");
@@ -486,8 +488,7 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     const int small_budget = std::min(lds_limit_ / 4, env_int("NRAPS_LDS_SMALL", lds_limit_ / 4));
     const int large_budget = std::min(lds_limit_ - 256, env_int("NRAPS_LDS_BUDGET", lds_limit_ - 256));
     refill_min_ = std::max(1, std::min(64, env_int("NRAPS_REFILL_MIN", 16)));
-    tab_lds_ = tab_max;
-    const int need = tab_max + tally_max;
+    const int need = need_max;
     variant_ = WcVariant{};
     if (need <= small_budget) {
       lds_tally_ = true;
@@ -497,7 +498,7 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     } else {
       lds_tally_ = false;
     }
-    lds_bytes_ = (size_t)tab_max + (lds_tally_ ? (size_t)tally_max : 0);
+    lds_bytes_ = (size_t)(lds_tally_ ? need_max : tab_max);
     variant_.fp32 = fp32_;
     variant_.lds_tally = lds_tally_;
     variant_.fission = fission_any_;
@@ -573,7 +574,7 @@ void Launcher::enqueue_cycle(hipStream_t s, hipEvent_t tr0, hipEvent_t tr1) {
   launch_cycle_begin(d_probs_, n_, max_bins_, s);
   if (tr0) HIP_TRY(hipEventRecord(tr0, s));
   if (woodcock_)
-    launch_woodcock(variant_, d_probs_, n_, grid_, lds_bytes_, tab_lds_, claim_, refill_min_, s);
+    launch_woodcock(variant_, d_probs_, n_, grid_, lds_bytes_, claim_, refill_min_, s);
   else
     launch_surface(d_probs_, n_, max_n_, s);
   if (tr1) HIP_TRY(hipEventRecord(tr1, s));
