@@ -635,6 +635,14 @@ __device__ inline double kint(const Ctx<Real> &C, int g, Real x, int c) {
   return T.kreg[g * (C.R + 1) + r] + T.nsfd[T.mat(c) + C.M * g] * ((double)x - T.regedge[r]);
 }
 
+// The same integral in region r of material m (what a region-tracked
+// particle holds): no cell-to-region or cell-to-material lookups.
+template <typename Real>
+__device__ inline double kint_rm(const Ctx<Real> &C, int g, Real x, int r, int m) {
+  const Tabs<Real> &T = C.T;
+  return T.kreg[g * (C.R + 1) + r] + T.nsfd[m + C.M * g] * ((double)x - T.regedge[r]);
+}
+
 // XS refresh: the particle's XS group gxs is taken in region r (material of
 // `cell`).  Staleness follows src/mc_code.rs:147: with the reference rule the
 // caller sets gxs = g only on entering another region.
@@ -648,12 +656,18 @@ __device__ inline void set_xs(const Ctx<Real> &C, Particle<Real> &q, int r, int 
 // Close the straight segment that started at (q.sx, q.scell) at (xe, ce):
 // its track-length tally and its k contribution, both with weight 1/|mu|.
 // The next segment starts there with k integral `kb`.
-template <typename Real, typename Acc>
+// RM: the end lies in the particle's XS region q.rxs (region tracking), so
+// the k integral takes that region and its material directly.
+#ifndef NRAPS_RS_KINT_RM
+#define NRAPS_RS_KINT_RM 1
+#endif
+template <typename Real, typename Acc, bool RM = false>
 __device__ inline void score_segment(const Ctx<Real> &C, Acc *direct, Acc *diff, Particle<Real> &q,
                                      Real xe, int ce) {
   const Real w = Num<Real>::rcp(fabs(q.mu));
   tally_piece<Real, Acc>(direct, diff, C.T.edges, q.g * C.nc, q.sx, q.scell, xe, ce, w);
-  const double kb = kint(C, q.g, xe, ce);
+  const double kb = (RM && NRAPS_RS_KINT_RM) ? kint_rm(C, q.g, xe, q.rxs, q.xs - C.M * q.gxs)
+                                             : kint(C, q.g, xe, ce);
   if (q.sx != xe) q.kacc += fabs(kb - q.ks) * (double)w;
   q.ks = kb;
   q.sx = xe;
@@ -760,7 +774,7 @@ __device__ inline uint32_t region_step(const Ctx<Real> &C, Acc *direct, Acc *dif
                                        Particle<Real> &q, uint32_t &ncross, int &nsite) {
   const Tabs<Real> &T = C.T;
   if (++q.events > C.max_events) {
-    score_segment<Real, Acc>(C, direct, diff, q, q.x, q.cell);
+    score_segment<Real, Acc, true>(C, direct, diff, q, q.x, q.cell);
     trace_event<F>(C, q, NRAPS_EV_CAP, q.cell, q.g, 0, q.x, q.mu);
     return kEvCap;
   }
@@ -774,7 +788,7 @@ __device__ inline uint32_t region_step(const Ctx<Real> &C, Acc *direct, Acc *dif
     ncross++;
     if (fwd ? r == C.R - 1 : r == 0) {
       const int fc = fwd ? C.nc - 1 : 0;
-      score_segment<Real, Acc>(C, direct, diff, q, edge, fc);
+      score_segment<Real, Acc, true>(C, direct, diff, q, edge, fc);
       const Real b = fwd ? C.br : C.bl;
       if (!(b > Real(0))) {
         trace_event<F>(C, q, NRAPS_EV_LEAK, fc, q.g, 0, edge, mu);
@@ -796,12 +810,13 @@ __device__ inline uint32_t region_step(const Ctx<Real> &C, Acc *direct, Acc *dif
       q.cell = locate(C, end, fwd);
       q.x = end;
     }
-    score_segment<Real, Acc>(C, direct, diff, q, q.x, q.cell);
+    score_segment<Real, Acc, true>(C, direct, diff, q, q.x, q.cell);
     const int g = q.g;
     if (!collide<F, Real>(C, q.xs, q.xs - C.M * q.gxs, rng, q, nsite))
       return kEvStep | kEvReal | kEvAbsorb;
     if (q.g != g) {
-      q.ks = kint(C, q.g, q.x, q.cell);
+      q.ks = NRAPS_RS_KINT_RM ? kint_rm(C, q.g, q.x, r, q.xs - C.M * q.gxs)
+                              : kint(C, q.g, q.x, q.cell);
       if (C.xs_every) {
         q.gxs = q.g;
         set_xs(C, q, r, q.cell);
