@@ -252,9 +252,20 @@ TabLayout Prepared::layout(bool fp32) const {
   L.bucket = put(2 * nb);
   L.regedge = put(8 * (R + 1));
   L.regcell = put(2 * (R + 1));
-  L.hot = at;
-  L.fuel = put(4 * nf);
-  L.dx = put(rs * nc);
+  // the birth tables join the LDS-staged part when they still leave the
+  // tables + tally within a quarter of the LDS (several 256-thread blocks
+  // per CU); larger problems read them from global memory
+  const int64_t birth = ((4 * (int64_t)nf + 15) & ~15) + ((rs * (int64_t)nc + 15) & ~15);
+  const int64_t tally = 2 * (int64_t)G * nc * rs;
+  if (kBirthInLds && at + birth + tally <= kLdsBytes / 4) {
+    L.fuel = put(4 * nf);
+    L.dx = put(rs * nc);
+    L.hot = at;
+  } else {
+    L.hot = at;
+    L.fuel = put(4 * nf);
+    L.dx = put(rs * nc);
+  }
   L.bytes = at;
   return L;
 }

@@ -42,13 +42,19 @@ struct TabLayout {
   int32_t bucket;   // u16[nb]      cell lookup: largest c with edge[c] < XL + b*h
   int32_t regedge;  // double[R+1]  region edges, regedge[r] = edges[regcell[r]]
   int32_t regcell;  // u16[R+1]     first cell of each region, regcell[R] = nc
-  int32_t hot;      // bytes of the LDS-staged part (all of the above)
+  int32_t hot;      // bytes of the LDS-staged part (all of the above, and the
+                    // birth tables when they are placed before `hot`)
   int32_t fuel;     // i32[nf]      source cells (birth only)
   int32_t dx;       // Real[nc]     Mesh.delta_x (birth only)
   int32_t bytes;    // total (multiple of 16)
 };
 
 constexpr int kMaxBuckets = 8192;
+constexpr int64_t kLdsBytes = 160 * 1024;  // LDS of one gfx950 workgroup
+#ifndef NRAPS_BIRTH_IN_LDS
+#define NRAPS_BIRTH_IN_LDS 1
+#endif
+constexpr bool kBirthInLds = NRAPS_BIRTH_IN_LDS != 0;
 constexpr int kMaxCells = 65535;
 
 struct Prepared {

@@ -144,8 +144,9 @@ struct Tabs {
     edges = (const Real *)(base + uni_if<U>(L.edges));
     mat8 = base + uni_if<U>(L.mat8);
     reg16 = (const uint16_t *)(base + uni_if<U>(L.reg16));
-    fuel = (const int32_t *)(global_base + uni_if<U>(L.fuel));
-    dx = (const Real *)(global_base + uni_if<U>(L.dx));
+    // birth tables: staged in LDS with the hot part when placed before it
+    fuel = (const int32_t *)((L.fuel < L.hot ? base : global_base) + uni_if<U>(L.fuel));
+    dx = (const Real *)((L.dx < L.hot ? base : global_base) + uni_if<U>(L.dx));
     sigw = (const Real *)(base + uni_if<U>(L.sigw));
     invtr = (const Real *)(base + uni_if<U>(L.invtr));
     absr = (const Real *)(base + uni_if<U>(L.absr));
