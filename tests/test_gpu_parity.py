@@ -125,3 +125,26 @@ def test_partial_albedo_faces_bit_exact(gpu, oracle, method):
     assert c.status_code == 0
     assert_traces_equal(g, c, o.histories)
     assert np.array_equal(g.k.view(np.uint64), c.k.view(np.uint64))
+
+
+@pytest.mark.parametrize("method", EVENT_METHODS)
+@pytest.mark.parametrize("faces", [(1.0, 1.0), (0.0, 0.0)])
+def test_single_region_and_two_material_slabs_bit_exact(gpu, oracle, method, faces):
+    """Edge shapes of the region tables: one material region (its edges are
+    the two faces; mirror faces bounce the remaining flight back and forth)
+    and the two-material slab of the hand-traced replay (a face region of
+    one cell), both FP64 event paths against the oracle history by history."""
+    from test_oracle_physics import homogeneous
+    from test_replay import two_material_slab
+    for p in (homogeneous(G=4, L=3.0, cells=7, bl=faces[0], br=faces[1])[0], two_material_slab()):
+        p.boundl, p.boundr = faces
+        o = nr.run_opts(histories=2000, generations=3, skip=1, method=method, precision=_abi.FP64,
+                        trace_histories=2000, trace_events=TRACE_EV, trace_cycle=2)
+        g = nr.solve(p, o, trace=True)
+        c = oracle.solve(p, o, threads=4, trace=True)
+        assert c.status_code == 0
+        assert_traces_equal(g, c, o.histories)
+        assert np.array_equal(g.k.view(np.uint64), c.k.view(np.uint64))
+        np.testing.assert_allclose(g.tally_last, c.tally_last, rtol=1e-10, atol=1e-10)
+        for f in ("collisions", "crossings", "absorptions", "leaks", "capped"):
+            assert getattr(g, f) == getattr(c, f), f
