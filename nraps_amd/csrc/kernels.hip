@@ -171,7 +171,8 @@ struct Ctx {
   Tabs<Real> T;
   int nc, G, M, R, nf, nb;
   int xs_every, fission, from_bank, make_sites, cycle;
-  int64_t max_events, N;
+  int32_t max_events;  // per-history cap, clamped to the range of the event counter
+  int64_t N;
   uint64_t seed, problem_id;
   Real maj, inv_maj, XL, XR, bl, br, inv_k, inv_h;
   double *kacc;
@@ -205,7 +206,7 @@ struct Ctx {
     from_bank = uni_if<U>(cs.from_bank);
     make_sites = uni_if<U>(cs.make_sites);
     cycle = uni_if<U>(cs.cycle);
-    max_events = uni_if<U>(P.max_events);
+    max_events = uni_if<U>((int32_t)(P.max_events < INT32_MAX - 1 ? P.max_events : INT32_MAX - 1));
     N = uni_if<U>(P.N);
     seed = uni_if<U>(P.seed);
     problem_id = uni_if<U>(P.problem_id);
@@ -788,9 +789,11 @@ __device__ inline uint32_t region_step(const Ctx<Real> &C, Acc *direct, Acc *dif
   const int r = q.rxs;
   const Real edge = (Real)T.regedge[fwd ? r + 1 : r];
   uint32_t ev = kEvStep;
-  if (fwd ? end > edge : edge > end) {
+  // both comparisons, combined without short-circuit branches
+  const bool past = (fwd & (end > edge)) | (!fwd & (edge > end));
+  if (past) {
     ncross++;
-    if (fwd ? r == C.R - 1 : r == 0) {
+    if ((fwd & (r == C.R - 1)) | (!fwd & (r == 0))) {
       const int fc = fwd ? C.nc - 1 : 0;
       score_segment<Real, Acc, true>(C, direct, diff, q, edge, fc);
       const Real b = fwd ? C.br : C.bl;
