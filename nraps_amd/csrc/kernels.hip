@@ -599,8 +599,12 @@ __device__ inline int locate(const Ctx<Real> &C, Real xe, bool right) {
   int b = (int)((xe - C.XL) * C.inv_h);
   b = b < 0 ? 0 : (b >= C.nb ? C.nb - 1 : b);
   int c = C.T.bucket[b];
-  while (c > 0 && !(right ? e[c] < xe : e[c] <= xe)) c--;
-  while (c + 1 < C.nc && (right ? e[c + 1] < xe : e[c + 1] <= xe)) c++;
+  // edge e below the point: e < xe moving right, e <= xe moving left; the
+  // conditions are combined without short-circuit branches (every index
+  // read is in 0..nc, edges has nc + 1 entries)
+  const bool left = !right;
+  while ((c > 0) & !((e[c] < xe) | (left & (e[c] == xe)))) c--;
+  while ((c + 1 < C.nc) & ((e[c + 1] < xe) | (left & (e[c + 1] == xe)))) c++;
   return c;
 }
 
