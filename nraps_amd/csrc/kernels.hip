@@ -813,7 +813,9 @@ __device__ inline uint32_t region_step(const Ctx<Real> &C, Acc *direct, Acc *dif
     }
     const int rn = fwd ? r + 1 : r - 1;
     q.x = edge;
-    q.cell = fwd ? (int)T.regcell[rn] : (int)T.regcell[r] - 1;
+    // first cell of region r + 1 moving right, last cell of region r - 1
+    // moving left: one load at a computed index, no branch
+    q.cell = (int)T.regcell[r + (int)fwd] - (int)!fwd;
     q.gxs = q.g;
     set_xs(C, q, rn, q.cell);
   } else {
