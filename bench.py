@@ -29,10 +29,10 @@ HBM_PEAK_GBS = 8000.0
 B_STATE_FP32, B_STATE_FP64 = 24, 32
 # HBM bytes per event-kernel launch of 10^7 histories on this workload, from
 # two rocprofv3 counter passes (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000,
-# factors calibrated on a 1 GiB device copy): region tracking 100.2 KiB x 2
-# read + 105933.5 KiB written (profiles/k_event_region_9region_4group_hbm_traffic.md),
+# factors calibrated on a 1 GiB device copy): region tracking 97.6 KiB x 2
+# read + 105869.3 KiB written (profiles/k_event_region_9region_4group_hbm_traffic.md),
 # Woodcock 15514.7 KiB x 2 + 159322.7 KiB (profiles/k_woodcock_9region_4group_hbm_traffic.md).
-TRAFFIC_BYTES_PER_LAUNCH = {"region": (100.21875 * 2 + 105933.5) * 1024,
+TRAFFIC_BYTES_PER_LAUNCH = {"region": (97.5625 * 2 + 105869.25) * 1024,
                             "woodcock": (15514.7 * 2 + 159322.7) * 1024}
 
 
