@@ -116,6 +116,8 @@ def solve_batch(problems, opts_list, *, check=True):
     ra = RA(*[C.pointer(r.c) for r in rbs])
     oa = OA(*opts_list)
     st = lib().nraps_mc_solve_batch(n, pa, oa, ra)
+    for rb in rbs:  # each problem's own status, as solve() reports it
+        rb.status_code = rb.c.status
     if check:
         _check(st)
     return rbs

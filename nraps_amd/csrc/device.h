@@ -49,6 +49,8 @@ struct DevProblem {
   int32_t nc, G, M, R, nf;
   int32_t xs_every, fission;
   int32_t dstart;       // fission bank started from the diffusion source
+  int32_t pass;         // which event-kernel launch of a cycle runs it: 0 with
+                        // the tally in LDS (or the only launch), 1 global
   int32_t tally_bins;  // 2*G*nc (direct + difference array)
   int32_t nb;          // cell-lookup buckets
   double inv_h;        // 1 / bucket width

@@ -970,7 +970,7 @@ constexpr int wc_waves_per_eu() {
 template <typename Real, bool LDS_TALLY, bool TRACE, bool FISSION, int BS, bool RS>
 __global__ __launch_bounds__(BS)
 __attribute__((amdgpu_waves_per_eu(wc_waves_per_eu<Real, TRACE, FISSION, BS, RS>())))
-void k_event(const DevProblem *probs, int nprob, int claim, int refill_min) {
+void k_event(const DevProblem *probs, int nprob, int claim, int refill_min, int pass) {
   using F = Feat<TRACE, FISSION>;
   extern __shared__ __align__(16) uint8_t smem[];
   __shared__ int s_has_work;
@@ -981,7 +981,8 @@ void k_event(const DevProblem *probs, int nprob, int claim, int refill_min) {
     const DevProblem &P = probs[pi];
     if (threadIdx.x == 0) {
       const CycleState *c = P.cs;
-      s_has_work = c->active && __hip_atomic_load(&P.cs->queue_head, __ATOMIC_RELAXED,
+      s_has_work = P.pass == pass && c->active &&
+                   __hip_atomic_load(&P.cs->queue_head, __ATOMIC_RELAXED,
                                                   __HIP_MEMORY_SCOPE_AGENT) < P.N;
     }
     __syncthreads();
@@ -1097,8 +1098,9 @@ void k_event(const DevProblem *probs, int nprob, int claim, int refill_min) {
 }
 
 #define NRAPS_WC_INST1(R, L, T, Fi, BS)                                                     \
-  template __global__ void k_event<R, L, T, Fi, BS, false>(const DevProblem *, int, int, int); \
-  template __global__ void k_event<R, L, T, Fi, BS, true>(const DevProblem *, int, int, int);
+  template __global__ void k_event<R, L, T, Fi, BS, false>(const DevProblem *, int, int, int, \
+                                                            int);                             \
+  template __global__ void k_event<R, L, T, Fi, BS, true>(const DevProblem *, int, int, int, int);
 #define NRAPS_WC_INST(R, L, BS)            \
   NRAPS_WC_INST1(R, L, false, false, BS) \
   NRAPS_WC_INST1(R, L, false, true, BS)  \
@@ -1444,9 +1446,10 @@ static const void *wc_select(const WcVariant &v) {
 }
 
 void launch_woodcock(const WcVariant &v, const DevProblem *d, int nprob, int grid, size_t lds_bytes,
-                     int claim, int refill_min, hipStream_t s) {
-  void *args[] = {(void *)&d, (void *)&nprob, (void *)&claim,
-                  (void *)&refill_min};
+                     int claim, int refill_min, int pass, hipStream_t s) {
+  // in the order of k_event's parameters
+  void *args[] = {(void *)&d, (void *)&nprob, (void *)&claim, (void *)&refill_min,
+                  (void *)&pass};
   hipError_t e = hipLaunchKernel(wc_select(v), dim3((unsigned)grid), dim3(v.block_size()), args,
                                  lds_bytes, s);
   (void)e;  // reported by the caller's hipGetLastError

@@ -20,7 +20,7 @@ struct WcVariant {
   int block_size() const { return big_block ? 1024 : 256; }
 };
 void launch_woodcock(const WcVariant &v, const DevProblem *d, int nprob, int grid, size_t lds_bytes,
-                     int claim, int refill_min, hipStream_t s);
+                     int claim, int refill_min, int pass, hipStream_t s);
 // Resident blocks per CU of the event kernel for this LDS size.
 int woodcock_blocks_per_cu(const WcVariant &v, size_t lds_bytes);
 hipError_t woodcock_set_lds_limit(const WcVariant &v, size_t lds_bytes);

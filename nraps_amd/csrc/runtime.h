@@ -124,18 +124,23 @@ class Launcher {
   void capture(hipStream_t s);
   void launch_graph(hipStream_t s);
   bool has_graph() const { return graph_exec_ != nullptr; }
-  int grid() const { return grid_; }
-  bool lds_tally() const { return lds_tally_; }
 
  private:
   void drop_graph();
   int device_;
-  int n_ = 0, n_cu_ = 0, lds_limit_ = 0, blocks_per_cu_ = 1, grid_ = 1, claim_ = 256, refill_min_ = 1;
+  int n_ = 0, n_cu_ = 0, lds_limit_ = 0, refill_min_ = 1;
   int max_bins_ = 0, levels_ = 1, max_groups_ = 1;
   int64_t max_n_ = 0, max_tiles_ = 0;
-  size_t lds_bytes_ = 0;
-  bool fp32_ = false, woodcock_ = false, fission_any_ = false, lds_tally_ = false;
-  WcVariant variant_{};
+  bool fp32_ = false, woodcock_ = false, fission_any_ = false;
+  // event-kernel launches of a cycle: [0] problems with the tally in LDS,
+  // [1] problems whose tables + tally do not fit (global FP64 tally)
+  struct EventPlan {
+    bool used = false;
+    WcVariant v{};
+    size_t lds_bytes = 0;
+    int grid = 1, claim = 256;
+  };
+  EventPlan plans_[2];
   std::vector<DevProblem> h_probs_;
   std::vector<DevFinal> h_fins_;
   std::vector<int64_t> h_tile_base_;

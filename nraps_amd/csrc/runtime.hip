@@ -453,7 +453,7 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
   max_groups_ = 1;
   levels_ = 1;
   max_tiles_ = 0;
-  int tab_max = 0, need_max = 0;  // largest tables; largest tables + tally of one problem
+  int tab_max = 0;  // largest tables of one problem
   bool trace_any = false;
   int64_t tiles_total = 0, total = 0;
   std::vector<int64_t> tile_base(std::max(n_, 1));
@@ -470,16 +470,18 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     tiles_total += t;
     max_tiles_ = std::max(max_tiles_, t);
     tab_max = std::max(tab_max, p.lay.hot);
-    need_max = std::max(need_max, p.lay.hot + p.dev.tally_bins * (fp32 ? 4 : 8));
     total += p.o.histories;
   }
+  std::vector<int32_t> pass(std::max(n_, 1), 0);
+  plans_[0] = plans_[1] = EventPlan{};
   if (woodcock_ && n_ > 0) {
     // LDS plan.  The hot tables always go to LDS.  The tally goes there too,
     // right after the tables of the problem a block works on: with
     // 256-thread blocks when the largest tables + tally of any one problem
     // leave room for four blocks per CU, else with one 1024-thread block per
-    // CU owning the whole LDS; only when neither fits is it scored into the
-    // global FP64 tally.
+    // CU owning the whole LDS.  Problems whose tables + tally exceed even
+    // that are run by a second launch of the cycle that scores into the
+    // global FP64 tally, so they do not take the rest of the set with them.
     if (tab_max > lds_limit_)
       throw std::invalid_argument("problem tables (" + std::to_string(tab_max) +
                                   " B) exceed the LDS of one block");
@@ -488,37 +490,45 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
     const int small_budget = std::min(lds_limit_ / 4, env_int("NRAPS_LDS_SMALL", lds_limit_ / 4));
     const int large_budget = std::min(lds_limit_ - 256, env_int("NRAPS_LDS_BUDGET", lds_limit_ - 256));
     refill_min_ = std::max(1, std::min(64, env_int("NRAPS_REFILL_MIN", 16)));
-    const int need = need_max;
-    variant_ = WcVariant{};
-    if (need <= small_budget) {
-      lds_tally_ = true;
-    } else if (need <= large_budget) {
-      lds_tally_ = true;
-      variant_.big_block = true;
-    } else {
-      lds_tally_ = false;
+    int need0 = 0, tab1 = 0;
+    int64_t total_pass[2] = {0, 0};
+    for (int i = 0; i < n_; i++) {
+      const ProblemDev &p = *ps[i];
+      const int need = p.lay.hot + p.dev.tally_bins * (fp32 ? 4 : 8);
+      pass[i] = need <= large_budget ? 0 : 1;
+      if (pass[i] == 0)
+        need0 = std::max(need0, need);
+      else
+        tab1 = std::max(tab1, p.lay.hot);
+      total_pass[pass[i]] += p.o.histories;
     }
-    lds_bytes_ = (size_t)(lds_tally_ ? need_max : tab_max);
-    variant_.fp32 = fp32_;
-    variant_.lds_tally = lds_tally_;
-    variant_.fission = fission_any_;
-    variant_.trace = trace_any;
-    variant_.region = ps[0]->o.method == NRAPS_EVENT_REGION;  // one method per batch
-    (void)woodcock_set_lds_limit(variant_, lds_bytes_);
-    blocks_per_cu_ = woodcock_blocks_per_cu(variant_, lds_bytes_);
-    grid_ = n_cu_ * blocks_per_cu_;
-    const int64_t waves = (int64_t)grid_ * (variant_.block_size() / 64);
-    int64_t c = total / (waves * 6);
-    // about 6 claims per wave and launch (measured best between 4 and 8 on
-    // the 9-region and 201-region slabs); the cap bounds the tail a last
-    // claim leaves at the end of the launch (forced 1024-history claims are
-    // 11% slower than automatic ones on a 10^7-history fission run) while
-    // large serving mixes (10^8 histories per launch) still get few atomics
-    // per wave (a cap of 256 there costs 16% of throughput)
-    const int claim_max = std::max(64, env_int("NRAPS_CLAIM_MAX", 1024));
-    c = std::max<int64_t>(64, std::min<int64_t>(claim_max, c));
-    claim_ = (int)((c + 63) / 64 * 64);
-    if (n_ == 1 && ps[0]->o.chunk > 0) claim_ = (int)std::min<int64_t>(ps[0]->o.chunk, 1 << 20);
+    for (int k = 0; k < 2; k++) {
+      EventPlan &P = plans_[k];
+      P.used = total_pass[k] > 0;
+      if (!P.used) continue;
+      P.v = WcVariant{};
+      P.v.lds_tally = k == 0;
+      P.v.big_block = k == 0 && need0 > small_budget;
+      P.lds_bytes = (size_t)(k == 0 ? need0 : tab1);
+      P.v.fp32 = fp32_;
+      P.v.fission = fission_any_;
+      P.v.trace = trace_any;
+      P.v.region = ps[0]->o.method == NRAPS_EVENT_REGION;  // one method per batch
+      (void)woodcock_set_lds_limit(P.v, P.lds_bytes);
+      P.grid = n_cu_ * woodcock_blocks_per_cu(P.v, P.lds_bytes);
+      const int64_t waves = (int64_t)P.grid * (P.v.block_size() / 64);
+      int64_t c = total_pass[k] / (waves * 6);
+      // about 6 claims per wave and launch (measured best between 4 and 8
+      // on the 9-region and 201-region slabs); the cap bounds the tail a
+      // last claim leaves at the end of the launch (forced 1024-history
+      // claims are 11% slower than automatic ones on a 10^7-history fission
+      // run) while large serving mixes (10^8 histories per launch) still get
+      // few atomics per wave (a cap of 256 there costs 16% of throughput)
+      const int claim_max = std::max(64, env_int("NRAPS_CLAIM_MAX", 1024));
+      c = std::max<int64_t>(64, std::min<int64_t>(claim_max, c));
+      P.claim = (int)((c + 63) / 64 * 64);
+      if (n_ == 1 && ps[0]->o.chunk > 0) P.claim = (int)std::min<int64_t>(ps[0]->o.chunk, 1 << 20);
+    }
   }
   if (n_ > cap_probs_) {
     if (d_probs_) (void)hipFree(d_probs_);
@@ -548,6 +558,7 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
   std::vector<DevFinal> df(n_);
   for (int i = 0; i < n_; i++) {
     dp[i] = ps[i]->dev;
+    dp[i].pass = pass[i];
     df[i] = ps[i]->fin;
   }
   if (n_ > 0) {
@@ -573,9 +584,13 @@ void Launcher::enqueue_cycle(hipStream_t s, hipEvent_t tr0, hipEvent_t tr1) {
   if (n_ == 0) return;
   launch_cycle_begin(d_probs_, n_, max_bins_, s);
   if (tr0) HIP_TRY(hipEventRecord(tr0, s));
-  if (woodcock_)
-    launch_woodcock(variant_, d_probs_, n_, grid_, lds_bytes_, claim_, refill_min_, s);
-  else
+  if (woodcock_) {
+    for (int k = 0; k < 2; k++) {
+      const EventPlan &P = plans_[k];
+      if (P.used)
+        launch_woodcock(P.v, d_probs_, n_, P.grid, P.lds_bytes, P.claim, refill_min_, k, s);
+    }
+  } else
     launch_surface(d_probs_, n_, max_n_, s);
   if (tr1) HIP_TRY(hipEventRecord(tr1, s));
   int64_t groups = max_n_;

@@ -40,6 +40,35 @@ def test_batch_equals_individual_solves(gpu):
         assert np.array_equal(s.sites, b.sites)
 
 
+@pytest.mark.parametrize("method", [_abi.EVENT_REGION, _abi.EVENT_WOODCOCK])
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_mixed_lds_and_global_tally_batch(gpu, oracle, method, use_graph):
+    """A batch whose largest problem cannot keep its tally in LDS (the
+    201-region 8-group slab in FP64: 256 KiB of tally) beside two that can:
+    the cycle runs two event-kernel launches, LDS tallies for the small
+    problems and the global FP64 tally for the large one.  Every problem's
+    k equals its solo solve and the oracle bit for bit, its tally the
+    oracle's to FP64 atomic summation order."""
+    big, _ = nr.generate(2)
+    small, _ = nr.generate(0)
+    kw = dict(generations=4, skip=2, method=method, precision=_abi.FP64,
+              source=_abi.SOURCE_FISSION_BANK, use_graph=use_graph)
+    probs = [small, big, small]
+    opts = [nr.run_opts(histories=3000, problem_id=0, **kw),
+            nr.run_opts(histories=4000, problem_id=1, **kw),
+            nr.run_opts(histories=2000, problem_id=2, **kw)]
+    print(f"mixed LDS/global batch: method {method}, graph {use_graph}", flush=True)
+    batch = nr.solve_batch(probs, opts)
+    for p, o, b in zip(probs, opts, batch):
+        assert b.status_code == 0
+        s = nr.solve(p, o)
+        c = oracle.solve(p, o, threads=4)
+        assert_same_k(s, b)
+        assert_same_k(c, b)
+        np.testing.assert_allclose(b.tally_last, c.tally_last, rtol=1e-10, atol=1e-10)
+        assert np.array_equal(s.sites, b.sites)
+
+
 def test_graph_replay_equals_direct_launches(gpu):
     p, o = small_c3(2)
     o.use_graph = 1
