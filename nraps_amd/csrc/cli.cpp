@@ -11,8 +11,9 @@
 // (src/main.rs:336-353): 1 runs the Monte Carlo solve, otherwise the
 // finite-difference diffusion solve (nalgebra_method) for `solver` 0; the
 // reference implements no other solver.  --mc / --diffusion override it.
-// Monte Carlo defaults: the deck's histories/generations/skip, FP32
-// event-based Woodcock tracking, the reference source and XS rules, seed 42.
+// Monte Carlo defaults: the deck's histories/generations/skip, the FP32
+// event kernel with region-by-region surface tracking (--event: Woodcock),
+// the reference source and XS rules, seed 42.
 // Writes vars.csv, interface.csv and k_eff.csv into OUTDIR (default ".").
 #include <cerrno>
 #include <cstdio>

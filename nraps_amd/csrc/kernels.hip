@@ -260,7 +260,7 @@ struct Feat {
 };
 using FeatAll = Feat<true, true>;
 
-// Outcome bits of one Woodcock step.
+// Outcome bits of one event-kernel step.
 enum : uint32_t {
   kEvStep = 1u,     // a flight was sampled
   kEvReal = 2u,     // real collision
@@ -277,7 +277,7 @@ struct Particle {
   int32_t p, events;
   int32_t nreal;  // real collisions of this history
   double kacc;
-  // Woodcock tracking scores a straight segment once, when it ends (real
+  // The event kernel scores a straight segment once, when it ends (real
   // collision, boundary, cap): its start point and the k integral there
   Real sx;
   int32_t scell;
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_surface(const DevProblem *probs, int
 }
 
 // ---------------------------------------------------------------------------
-// Event pipeline with Woodcock delta tracking
+// Event pipeline: region surface tracking and Woodcock delta tracking
 // ---------------------------------------------------------------------------
 // Cell holding the end of a move: moving right (mu >= 0) a point on an edge
 // belongs to the cell on its left, moving left to the cell on its right
@@ -1445,7 +1445,7 @@ static const void *wc_select(const WcVariant &v) {
   return v.fp32 ? wc_pick_mode<float>(v) : wc_pick_mode<double>(v);
 }
 
-void launch_woodcock(const WcVariant &v, const DevProblem *d, int nprob, int grid, size_t lds_bytes,
+void launch_event(const WcVariant &v, const DevProblem *d, int nprob, int grid, size_t lds_bytes,
                      int claim, int refill_min, int pass, hipStream_t s) {
   // in the order of k_event's parameters
   void *args[] = {(void *)&d, (void *)&nprob, (void *)&claim, (void *)&refill_min,
@@ -1455,11 +1455,11 @@ void launch_woodcock(const WcVariant &v, const DevProblem *d, int nprob, int gri
   (void)e;  // reported by the caller's hipGetLastError
 }
 
-hipError_t woodcock_set_lds_limit(const WcVariant &v, size_t lds_bytes) {
+hipError_t event_set_lds_limit(const WcVariant &v, size_t lds_bytes) {
   return hipFuncSetAttribute(wc_select(v), hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
 }
 
-int woodcock_blocks_per_cu(const WcVariant &v, size_t lds_bytes) {
+int event_blocks_per_cu(const WcVariant &v, size_t lds_bytes) {
   int nb = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, wc_select(v), v.block_size(), lds_bytes) !=
       hipSuccess)

@@ -19,11 +19,11 @@ struct WcVariant {
   bool region = false;    // region-by-region surface tracking (else Woodcock)
   int block_size() const { return big_block ? 1024 : 256; }
 };
-void launch_woodcock(const WcVariant &v, const DevProblem *d, int nprob, int grid, size_t lds_bytes,
+void launch_event(const WcVariant &v, const DevProblem *d, int nprob, int grid, size_t lds_bytes,
                      int claim, int refill_min, int pass, hipStream_t s);
 // Resident blocks per CU of the event kernel for this LDS size.
-int woodcock_blocks_per_cu(const WcVariant &v, size_t lds_bytes);
-hipError_t woodcock_set_lds_limit(const WcVariant &v, size_t lds_bytes);
+int event_blocks_per_cu(const WcVariant &v, size_t lds_bytes);
+hipError_t event_set_lds_limit(const WcVariant &v, size_t lds_bytes);
 void launch_ktree(const DevProblem *d, int nprob, int level, int64_t max_groups, hipStream_t s);
 void launch_bank(bool fp32, const DevProblem *d, int nprob, int64_t max_n, int64_t max_tiles,
                  int64_t *tile_sums, const int64_t *tile_base, hipStream_t s);

@@ -68,12 +68,12 @@ class ProblemDev {
   // not null (host sources are owned by this object or by the caller's
   // problem, which must stay alive until the stream has run them).
   // cs_slot: externally owned CycleState (server slot array) or nullptr.
-  void upload(bool woodcock, CycleState *cs_slot, bool want_trace, nraps_result *trace_res,
+  void upload(bool event, CycleState *cs_slot, bool want_trace, nraps_result *trace_res,
               hipStream_t s);
-  void setup(const nraps_problem *pb, const nraps_run_opts &o, bool fp32, bool woodcock,
+  void setup(const nraps_problem *pb, const nraps_run_opts &o, bool fp32, bool event,
              CycleState *cs_slot, bool want_trace, nraps_result *trace_res, hipStream_t s) {
     prepare(pb, o, fp32);
-    upload(woodcock, cs_slot, want_trace, trace_res, s);
+    upload(event, cs_slot, want_trace, trace_res, s);
   }
   // Copy results into r (one synchronisation of the set-up stream); returns
   // the status of the run.
@@ -117,7 +117,7 @@ class Launcher {
   ~Launcher();
   // Launch plan for a set of problems; the tables go to the device on `s`
   // (ordered after the work already queued there) when it is not null.
-  void build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodcock, hipStream_t s = nullptr);
+  void build(const std::vector<ProblemDev *> &ps, bool fp32, bool event, hipStream_t s = nullptr);
   // Launches of one generation for every problem of the set.
   void enqueue_cycle(hipStream_t s, hipEvent_t tr0, hipEvent_t tr1);
   // Capture one generation into a graph (replaces any previous capture).
@@ -131,7 +131,7 @@ class Launcher {
   int n_ = 0, n_cu_ = 0, lds_limit_ = 0, refill_min_ = 1;
   int max_bins_ = 0, levels_ = 1, max_groups_ = 1;
   int64_t max_n_ = 0, max_tiles_ = 0;
-  bool fp32_ = false, woodcock_ = false, fission_any_ = false;
+  bool fp32_ = false, event_ = false, fission_any_ = false;
   // event-kernel launches of a cycle: [0] problems with the tally in LDS,
   // [1] problems whose tables + tally do not fit (global FP64 tally)
   struct EventPlan {

@@ -109,7 +109,7 @@ void ProblemDev::prepare(const nraps_problem *pb_, const nraps_run_opts &o_, boo
   for (int c = 0; c < prep.nc; c++) width[c] = pb->mesh_right[c] - pb->mesh_left[c];
 }
 
-void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nraps_result *r,
+void ProblemDev::upload(bool event, CycleState *cs_slot, bool want_trace, nraps_result *r,
                         hipStream_t s) {
   stream = s;
   if (s) arena.bind(s);
@@ -297,7 +297,7 @@ void ProblemDev::upload(bool woodcock, CycleState *cs_slot, bool want_trace, nra
   f.k_ref = reinterpret_cast<double *>(base + B.kref);
   f.sites_hist = reinterpret_cast<int64_t *>(base + B.sites);
   f.kpart = reinterpret_cast<double *>(base + B.kpart);
-  f.has_diff = woodcock ? 1 : 0;
+  f.has_diff = event ? 1 : 0;
 }
 
 void ProblemDev::deactivate(hipStream_t s) {
@@ -442,11 +442,11 @@ void Launcher::drop_graph() {
   graph_ = nullptr;
 }
 
-void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodcock, hipStream_t s) {
+void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool event, hipStream_t s) {
   drop_graph();
   n_ = (int)ps.size();
   fp32_ = fp32;
-  woodcock_ = woodcock;
+  event_ = event;
   fission_any_ = false;
   max_n_ = 0;
   max_bins_ = 0;
@@ -474,7 +474,7 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
   }
   std::vector<int32_t> pass(std::max(n_, 1), 0);
   plans_[0] = plans_[1] = EventPlan{};
-  if (woodcock_ && n_ > 0) {
+  if (event_ && n_ > 0) {
     // LDS plan.  The hot tables always go to LDS.  The tally goes there too,
     // right after the tables of the problem a block works on: with
     // 256-thread blocks when the largest tables + tally of any one problem
@@ -514,8 +514,8 @@ void Launcher::build(const std::vector<ProblemDev *> &ps, bool fp32, bool woodco
       P.v.fission = fission_any_;
       P.v.trace = trace_any;
       P.v.region = ps[0]->o.method == NRAPS_EVENT_REGION;  // one method per batch
-      (void)woodcock_set_lds_limit(P.v, P.lds_bytes);
-      P.grid = n_cu_ * woodcock_blocks_per_cu(P.v, P.lds_bytes);
+      (void)event_set_lds_limit(P.v, P.lds_bytes);
+      P.grid = n_cu_ * event_blocks_per_cu(P.v, P.lds_bytes);
       const int64_t waves = (int64_t)P.grid * (P.v.block_size() / 64);
       int64_t c = total_pass[k] / (waves * 6);
       // about 6 claims per wave and launch (measured best between 4 and 8
@@ -584,11 +584,11 @@ void Launcher::enqueue_cycle(hipStream_t s, hipEvent_t tr0, hipEvent_t tr1) {
   if (n_ == 0) return;
   launch_cycle_begin(d_probs_, n_, max_bins_, s);
   if (tr0) HIP_TRY(hipEventRecord(tr0, s));
-  if (woodcock_) {
+  if (event_) {
     for (int k = 0; k < 2; k++) {
       const EventPlan &P = plans_[k];
       if (P.used)
-        launch_woodcock(P.v, d_probs_, n_, P.grid, P.lds_bytes, P.claim, refill_min_, k, s);
+        launch_event(P.v, d_probs_, n_, P.grid, P.lds_bytes, P.claim, refill_min_, k, s);
     }
   } else
     launch_surface(d_probs_, n_, max_n_, s);
@@ -664,7 +664,7 @@ int solve_batch(int n, const nraps_problem *const *pbs, const nraps_run_opts *op
     }
     HIP_TRY(hipSetDevice(device));
     const bool fp32 = opts[0].precision == NRAPS_FP32;
-    const bool woodcock = opts[0].method != NRAPS_HISTORY_SURFACE;  // event pipeline
+    const bool event = opts[0].method != NRAPS_HISTORY_SURFACE;  // event pipeline
     const bool use_graph = opts[0].use_graph != 0;
     // the stream outlives the problems: their memory is freed on it
     StreamGuard sg;
@@ -674,13 +674,13 @@ int solve_batch(int n, const nraps_problem *const *pbs, const nraps_run_opts *op
     int gens_max = 0;
     for (int i = 0; i < n; i++) {
       owned[i].reset(new ProblemDev());
-      owned[i]->setup(pbs[i], opts[i], fp32, woodcock, nullptr, true, results ? results[i] : nullptr,
+      owned[i]->setup(pbs[i], opts[i], fp32, event, nullptr, true, results ? results[i] : nullptr,
                       sg.s);
       ps[i] = owned[i].get();
       gens_max = std::max(gens_max, (int)opts[i].generations);
     }
     Launcher L(device);
-    L.build(ps, fp32, woodcock, sg.s);
+    L.build(ps, fp32, event, sg.s);
     hipEvent_t e0 = sg.make(), e1 = sg.make();
     std::vector<hipEvent_t> tr;
     HIP_TRY(hipEventRecord(e0, sg.s));

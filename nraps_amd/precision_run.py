@@ -2,8 +2,8 @@
 
 Runs the workload twice on the GPU with the same seed: FP32 tracking and
 tallies (FP64 k, bank and variance sums) and the FP64 event path. Both use
-Woodcock tracking and the fission-site bank, and the histories follow the
-same PCG32 streams. It reports the k of each (mean ± standard error over the
+the workload's event tracking (region-by-region surface tracking) and the
+fission-site bank, and the histories follow the same PCG32 streams. It reports the k of each (mean ± standard error over the
 active generations), their difference in combined standard errors, the
 relative differences of the eight group-total fluxes, and per-bin agreement
 of the active-mean flux (fraction of bins within 3 combined standard errors,
