@@ -29,8 +29,12 @@
  *   - 32-bit table indices (the reference computes them in u8);
  *   - the birth position is clamped to the cell's right edge;
  *   - the source cell is drawn by one multiply-shift draw.
- * New behaviour (not in the reference): Woodcock tracking, the fission-site
- * bank, per-history k accumulation with a canonical summation order.
+ * New behaviour (not in the reference): region-by-region surface tracking
+ * (ohistory_region: the reference's tracking with the cell crossings inside a
+ * material region merged; it draws and scores like ohistory_surface up to
+ * rounding, which tests/test_region.py checks history by history), Woodcock
+ * tracking, the fission-site bank, per-history k accumulation with a
+ * canonical summation order.
  */
 #define _GNU_SOURCE
 #include "oracle.h"
