@@ -115,7 +115,8 @@ typedef struct nraps_run_opts {
   int32_t source;     /* NRAPS_SOURCE_* */
   int32_t xs_refresh; /* NRAPS_XS_* */
   double k_initial;   /* the k_new argument of monte_carlo (reference: 1.0) */
-  int64_t max_events; /* per-history event cap; 0 => 10^7 */
+  int64_t max_events; /* per-history event cap; 0 => 10^7; caps above 2^31 - 2
+                         act as 2^31 - 2 (the width of the event counter) */
   int32_t device;     /* HIP device ordinal */
   int32_t use_graph;  /* capture one cycle in a hipGraph and replay it */
   int64_t chunk;      /* particles per work chunk; 0 => automatic */
