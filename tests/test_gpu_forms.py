@@ -211,10 +211,11 @@ def _seed_stats(run, seeds):
 
 
 def test_statistical_agreement_9region_slab(gpu, oracle):
-    """FP32 event, FP64 event and the plain FP64 port against the FP64
-    oracle on the 9-region 4-group slab, eight independent seeds each: k and the group
-    fluxes within 4 combined standard errors and 0.5%; the FP32 and FP64
-    event paths on the same streams agree to 1e-4."""
+    """FP32 and FP64 event kernels (Woodcock and region tracking) and the
+    plain FP64 port against the FP64 oracle on the 9-region 4-group slab,
+    eight independent seeds each: k and the group fluxes within 4 combined
+    standard errors and 0.5%; the FP32 and FP64 event paths of one tracker
+    on the same streams agree to 1e-4."""
     p, _ = nr.generate(0)
     seeds = range(101, 109)
     ref = _seed_stats(lambda s: oracle.solve(
@@ -222,6 +223,7 @@ def test_statistical_agreement_9region_slab(gpu, oracle):
     modes = {"fp32 event": dict(method=_abi.EVENT_WOODCOCK, precision=_abi.FP32),
              "fp64 event": dict(method=_abi.EVENT_WOODCOCK, precision=_abi.FP64),
              "fp32 region": dict(method=_abi.EVENT_REGION, precision=_abi.FP32),
+             "fp64 region": dict(method=_abi.EVENT_REGION, precision=_abi.FP64),
              "fp64 plain": dict(method=_abi.HISTORY_SURFACE, precision=_abi.FP64)}
     got = {}
     for name, kw in modes.items():
@@ -230,17 +232,19 @@ def test_statistical_agreement_9region_slab(gpu, oracle):
         assert abs(st[0] - ref[0]) < 4 * math.hypot(st[1], ref[1]), name
         assert np.all(np.abs(st[2] - ref[2]) < 4 * np.hypot(st[3], ref[3])), name
         assert np.all(np.abs(st[2] / ref[2] - 1) < 5e-3), name
-    a, b = got["fp32 event"], got["fp64 event"]
-    assert abs(a[0] / b[0] - 1) < 1e-4 and np.all(np.abs(a[2] / b[2] - 1) < 1e-4)
+    for f32, f64 in (("fp32 event", "fp64 event"), ("fp32 region", "fp64 region")):
+        a, b = got[f32], got[f64]
+        assert abs(a[0] / b[0] - 1) < 1e-4 and np.all(np.abs(a[2] / b[2] - 1) < 1e-4), f32
 
 
-def test_precision_form_201region_fp32_vs_fp64(gpu):
+@pytest.mark.parametrize("method", [_abi.EVENT_REGION, _abi.EVENT_WOODCOCK])
+def test_precision_form_201region_fp32_vs_fp64(gpu, method):
     """Precision form on the 201-region 8-group fission-bank slab: FP32
     tracking/tallies with FP64 k and bank against the FP64 event path, four
     independent seeds each: k and the eight group fluxes within 4 combined
     standard errors, group fluxes within 0.5%."""
     p, _ = nr.generate(2)
-    kw = dict(histories=2 * 10**6, generations=14, skip=4, method=_abi.EVENT_WOODCOCK,
+    kw = dict(histories=2 * 10**6, generations=14, skip=4, method=method,
               source=_abi.SOURCE_FISSION_BANK)
     stats = {}
     for prec in (_abi.FP64, _abi.FP32):
@@ -273,7 +277,8 @@ def test_full_size_9region_1e7_invariants(gpu):
     assert abs(r64.k_mean - r.k_mean) < 5 * math.hypot(r.k_sigma, r64.k_sigma)
 
 
-def test_fp32_tally_reduction_error(gpu, monkeypatch):
+@pytest.mark.parametrize("method", [_abi.EVENT_REGION, _abi.EVENT_WOODCOCK])
+def test_fp32_tally_reduction_error(gpu, monkeypatch, method):
     """Error of the FP32 tally reduction, isolated: the same FP32 trajectories
     scored (a) as FP32 partials in each block's LDS, flushed into the FP64
     generation tally (the production path) and (b) straight into the FP64
@@ -281,7 +286,7 @@ def test_fp32_tally_reduction_error(gpu, monkeypatch):
     must be bit-identical; the last generation's tally agrees to 1e-5 of
     each group's largest bin, and so do the active-mean fluxes."""
     p, _ = nr.generate(0)
-    kw = dict(histories=10**6, generations=4, skip=1, method=_abi.EVENT_WOODCOCK,
+    kw = dict(histories=10**6, generations=4, skip=1, method=method,
               precision=_abi.FP32, use_graph=0)
     a = nr.solve(p, nr.run_opts(**kw))
     monkeypatch.setenv("NRAPS_LDS_SMALL", "0")
